@@ -1,0 +1,352 @@
+"""genfft_amd -- MI355X (gfx950) FFT engine with genFFT's plan-object surface.
+
+The compute path is the hand-written HIP library ``libgenfft_amd.so`` behind
+the C ABI in ``include/genfft_amd.h``; this module is a thin ctypes layer that
+takes PyTorch tensors (device memory and the current stream), mirroring the
+reference classes for Python callers:
+
+    genfft::FFT<T>         include/genFFT/fft.h:56-113      -> FFT
+    genfft::FFTVert<T>     include/genFFT/fft.h:117-171     -> FFTVert
+    genfft::DIT<T>         include/genFFT/fft.h:173-196     -> DIT
+    genfft::FFT2D<T>       include/genFFT/fft.h:200-245     -> FFT2D
+    genfft::RealFFT<T>     include/genFFT/FFTReal.h:188-221 -> RealFFT
+    genfft::RealFFT2D<T>   include/genFFT/FFTReal.h:71-184  -> RealFFT2D
+    separate_2x_real_FFT   include/genFFT/FFTReal.h:36-66   -> separate_2x_real_FFT
+
+Same conventions as the reference: forward kernel exp(-2*pi*i*j*k/N),
+unscaled inverse, natural order, power-of-two sizes.  Strides and row
+pitches are in complex elements (RealFFT2D input pitch in scalars), exactly as
+in the reference API.  Tensors must live on a GPU; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import torch
+
+from . import _capi as C
+from ._capi import GfftError
+
+__all__ = [
+    "Plan", "FFT", "FFTVert", "DIT", "FFT2D", "RealFFT", "RealFFT2D",
+    "separate_2x_real_FFT", "fft", "ifft", "rfft", "irfft", "fft2", "GfftError",
+]
+
+_COMPLEX = {torch.complex64: C.GFFT_F32, torch.complex128: C.GFFT_F64}
+_REAL = {torch.float32: C.GFFT_F32, torch.float64: C.GFFT_F64}
+_REAL_OF = {torch.complex64: torch.float32, torch.complex128: torch.float64}
+_COMPLEX_OF = {torch.float32: torch.complex64, torch.float64: torch.complex128}
+
+
+def _prec(dtype: torch.dtype) -> int:
+    if dtype in _COMPLEX:
+        return _COMPLEX[dtype]
+    if dtype in _REAL:
+        return _REAL[dtype]
+    raise TypeError(f"unsupported dtype {dtype}: use float32/float64 or complex64/complex128")
+
+
+def _check_gpu(*ts: torch.Tensor) -> torch.device:
+    dev = None
+    for t in ts:
+        if not t.is_cuda:
+            raise ValueError("genfft_amd works on GPU tensors only (got a CPU tensor)")
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise ValueError("all tensors must be on the same device")
+    return dev
+
+
+class Plan:
+    """One gfft_plan (include/genfft_amd.h); owns the plan's device workspace."""
+
+    def __init__(self, kind: int, n: int, *, precision: int = C.GFFT_F32, n2: int = 0,
+                 batch: int = 1, in_stride: int = 0, out_stride: int = 0, in_dist: int = 0,
+                 out_dist: int = 0, half: bool = False, device=None):
+        self.desc = C.GfftDesc(kind, precision, n, n2, batch, in_stride, out_stride, in_dist,
+                               out_dist, C.GFFT_FLAG_HALF if half else 0, 0)
+        self._h = ctypes.c_void_p()
+        self._lib = C.lib()
+        if device is not None:
+            with torch.cuda.device(device):
+                C.check(self._lib.gfft_plan_create(ctypes.byref(self._h), ctypes.byref(self.desc)))
+        else:
+            C.check(self._lib.gfft_plan_create(ctypes.byref(self._h), ctypes.byref(self.desc)))
+
+    def execute(self, inp: torch.Tensor, out: torch.Tensor, inverse: bool = False, stream=None) -> None:
+        dev = _check_gpu(inp, out)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        with torch.cuda.device(dev):
+            C.check(self._lib.gfft_execute(self._h, inp.data_ptr(), out.data_ptr(),
+                                           C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
+                                           s.cuda_stream))
+
+    def execute_ptr(self, in_ptr: int, out_ptr: int, inverse: bool = False, stream_handle: int = 0) -> None:
+        C.check(self._lib.gfft_execute(self._h, in_ptr, out_ptr,
+                                       C.GFFT_INVERSE if inverse else C.GFFT_FORWARD, stream_handle))
+
+    def profile(self, inp: torch.Tensor, out: torch.Tensor, inverse: bool = False, stream=None):
+        """Runs once, returning the per-launch kernel durations in ms (HIP events
+        recorded on the execution stream around each launch)."""
+        dev = _check_gpu(inp, out)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        ms = (ctypes.c_float * 4096)()
+        cnt = ctypes.c_int()
+        with torch.cuda.device(dev):
+            C.check(self._lib.gfft_plan_profile(self._h, inp.data_ptr(), out.data_ptr(),
+                                                C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
+                                                s.cuda_stream, ms, len(ms), ctypes.byref(cnt)))
+        return [ms[i] for i in range(min(cnt.value, len(ms)))]
+
+    def workspace_bytes(self) -> int:
+        v = ctypes.c_size_t()
+        C.check(self._lib.gfft_plan_workspace(self._h, ctypes.byref(v)))
+        return v.value
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(8192)
+        C.check(self._lib.gfft_plan_describe(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.gfft_plan_destroy(h)
+            self._h = None
+
+
+# Plan cache: the analogue of the reference's per-(N, T) singleton
+# (include/genFFT/FFTLevel.h:76-92), keyed by the full descriptor and device,
+# and guarded by a lock (the reference's cache is not thread-safe).
+_cache: dict = {}
+_cache_lock = threading.Lock()
+
+
+def _plan(kind, n, precision, device, **kw) -> Plan:
+    key = (kind, n, precision, device.index if device.index is not None else torch.cuda.current_device(),
+           tuple(sorted(kw.items())))
+    with _cache_lock:
+        p = _cache.get(key)
+        if p is None:
+            p = Plan(kind, n, precision=precision, device=device, **kw)
+            _cache[key] = p
+        return p
+
+
+def _batch_of(t: torch.Tensor, n: int) -> int:
+    if t.numel() % n:
+        raise ValueError(f"tensor of {t.numel()} elements is not a whole number of length-{n} rows")
+    return max(1, t.numel() // n)
+
+
+def _need_contig(*ts):
+    for t in ts:
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+
+
+class FFT:
+    """genfft::FFT<T>(n): batched 1D complex transforms over the last axis."""
+
+    def __init__(self, n: int = 0, dtype: torch.dtype = torch.complex64):
+        self.n = int(n)
+        self.dtype = dtype
+        self._prec = _prec(dtype)
+
+    def size(self) -> int:
+        return self.n
+
+    def __bool__(self) -> bool:
+        return self.n > 0
+
+    def transform(self, out: torch.Tensor, inp: torch.Tensor, inv: bool = False) -> None:
+        """fft.h:80-85: natural-order transform, unscaled inverse."""
+        dev = _check_gpu(out, inp)
+        _need_contig(out, inp)
+        b = _batch_of(inp, self.n)
+        _plan(C.GFFT_C2C, self.n, self._prec, dev, batch=b).execute(inp, out, inverse=inv)
+
+    def forward(self, out, inp):
+        self.transform(out, inp, False)
+
+    def inverse(self, out, inp):
+        self.transform(out, inp, True)
+
+    def transform_real(self, out: torch.Tensor, inp: torch.Tensor) -> None:
+        """fft.h:90-94: real input, full n-bin spectrum."""
+        dev = _check_gpu(out, inp)
+        _need_contig(out, inp)
+        b = _batch_of(inp, self.n)
+        _plan(C.GFFT_R2C, self.n, self._prec, dev, batch=b).execute(inp, out)
+
+    def transform_interleave(self, out: torch.Tensor, in1: torch.Tensor, in2: torch.Tensor) -> None:
+        """fft.h:100-105: FFT of in1 + i*in2 (split later with separate_2x_real_FFT)."""
+        dev = _check_gpu(out, in1, in2)
+        _need_contig(out, in1, in2)
+        b = _batch_of(in1, self.n)
+        lib = C.lib()
+        s = torch.cuda.current_stream(dev).cuda_stream
+        with torch.cuda.device(dev):
+            C.check(lib.gfft_interleave_real(self._prec, out.data_ptr(), in1.data_ptr(), in2.data_ptr(),
+                                             self.n, b, s))
+        _plan(C.GFFT_C2C, self.n, self._prec, dev, batch=b).execute(out, out)
+
+
+class RealFFT:
+    """genfft::RealFFT<T>(n).forward(out, in, half) (FFTReal.h:188-221)."""
+
+    def __init__(self, n: int = 0, dtype: torch.dtype = torch.float32):
+        self.n = int(n)
+        self._prec = _prec(dtype)
+
+    def size(self) -> int:
+        return self.n
+
+    def forward(self, out: torch.Tensor, inp: torch.Tensor, half: bool) -> None:
+        dev = _check_gpu(out, inp)
+        _need_contig(out, inp)
+        b = _batch_of(inp, self.n)
+        nout = self.n // 2 + 1 if half else self.n
+        if out.numel() < b * nout:
+            raise ValueError("output too small")
+        od = out.numel() // b
+        _plan(C.GFFT_R2C, self.n, self._prec, dev, batch=b, half=half, out_dist=od).execute(inp, out)
+
+    def inverse(self, out: torch.Tensor, inp: torch.Tensor) -> None:
+        """Half spectrum (n/2+1 bins) -> n reals, unscaled (the reference's
+        RealFFT2D::inverse is a TODO, FFTReal.h:127-130; this is the 1D c2r)."""
+        dev = _check_gpu(out, inp)
+        _need_contig(out, inp)
+        b = _batch_of(out, self.n)
+        _plan(C.GFFT_C2R, self.n, self._prec, dev, batch=b).execute(inp, out, inverse=True)
+
+
+class DIT:
+    """genfft::DIT<T>(n).apply(out, in, half): the real-FFT unpack step."""
+
+    def __init__(self, n: int = 0, dtype: torch.dtype = torch.complex64):
+        self.n = int(n)
+        self._prec = _prec(dtype)
+
+    def size(self) -> int:
+        return self.n
+
+    def apply(self, out: torch.Tensor, inp: torch.Tensor, half: bool) -> None:
+        dev = _check_gpu(out, inp)
+        nin = max(1, self.n // 2)
+        b = _batch_of(inp, nin)
+        od = out.numel() // b
+        _plan(C.GFFT_DIT, self.n, self._prec, dev, batch=b, half=half, out_dist=od).execute(inp, out)
+
+
+class FFTVert:
+    """genfft::FFTVert<T>(n): n-point FFTs down `cols` columns (fft.h:117-171)."""
+
+    def __init__(self, n: int = 0, dtype: torch.dtype = torch.complex64):
+        self.n = int(n)
+        self._prec = _prec(dtype)
+
+    def size(self) -> int:
+        return self.n
+
+    def transform(self, out, out_stride, inp, in_stride=None, cols=None, inv: bool = False) -> None:
+        if in_stride is None:  # transform(out, in, cols) overload, fft.h:158-163
+            out, inp, cols = out, out_stride, inp
+            out_stride = in_stride = cols
+        dev = _check_gpu(out, inp)
+        _plan(C.GFFT_COLS, self.n, self._prec, dev, n2=int(cols), in_stride=int(in_stride),
+              out_stride=int(out_stride)).execute(inp, out, inverse=inv)
+
+
+class FFT2D:
+    """genfft::FFT2D<T>(width, height) (fft.h:200-245)."""
+
+    def __init__(self, width: int = 0, height: int = 0, dtype: torch.dtype = torch.complex64):
+        self.width, self.height = int(width), int(height)
+        self._prec = _prec(dtype)
+
+    def cols(self) -> int:
+        return self.width
+
+    def rows(self) -> int:
+        return self.height
+
+    def __bool__(self) -> bool:
+        return self.width > 0 and self.height > 0
+
+    def transform(self, out, out_stride, inp, in_stride, inv: bool = False, batch: int = 1) -> None:
+        dev = _check_gpu(out, inp)
+        _plan(C.GFFT_C2C_2D, self.width, self._prec, dev, n2=self.height, in_stride=int(in_stride),
+              out_stride=int(out_stride), batch=batch).execute(inp, out, inverse=inv)
+
+
+class RealFFT2D:
+    """genfft::RealFFT2D<T>(width, height).forward (FFTReal.h:71-184)."""
+
+    def __init__(self, width: int = 0, height: int = 0, dtype: torch.dtype = torch.float32):
+        self.width, self.height = int(width), int(height)
+        self._prec = _prec(dtype)
+
+    def cols(self) -> int:
+        return self.width
+
+    def rows(self) -> int:
+        return self.height
+
+    def forward(self, out, out_stride, inp, in_stride) -> None:
+        dev = _check_gpu(out, inp)
+        _plan(C.GFFT_R2C_2D, self.width, self._prec, dev, n2=self.height, in_stride=int(in_stride),
+              out_stride=int(out_stride)).execute(inp, out)
+
+
+def separate_2x_real_FFT(out1: torch.Tensor, out2: torch.Tensor, inp: torch.Tensor, n: int) -> None:
+    """FFTReal.h:36-66 on device data (batched over rows of n)."""
+    dev = _check_gpu(out1, out2, inp)
+    b = _batch_of(inp, n)
+    s = torch.cuda.current_stream(dev).cuda_stream
+    with torch.cuda.device(dev):
+        C.check(C.lib().gfft_separate_2x_real(_prec(inp.dtype), out1.data_ptr(), out2.data_ptr(),
+                                              inp.data_ptr(), n, b, s))
+
+
+# ---------------------------------------------------------------------------
+# functional helpers over the last axis (allocate the output)
+# ---------------------------------------------------------------------------
+def fft(x: torch.Tensor, inverse: bool = False) -> torch.Tensor:
+    x = x.contiguous()
+    out = torch.empty_like(x)
+    FFT(x.shape[-1], x.dtype).transform(out, x, inverse)
+    return out
+
+
+def ifft(x: torch.Tensor) -> torch.Tensor:
+    """Unscaled inverse (genFFT convention)."""
+    return fft(x, inverse=True)
+
+
+def rfft(x: torch.Tensor, half: bool = True) -> torch.Tensor:
+    x = x.contiguous()
+    n = x.shape[-1]
+    out = torch.empty(*x.shape[:-1], n // 2 + 1 if half else n, dtype=_COMPLEX_OF[x.dtype], device=x.device)
+    RealFFT(n, x.dtype).forward(out, x, half)
+    return out
+
+
+def irfft(X: torch.Tensor, n: int) -> torch.Tensor:
+    """Unscaled c2r: n reals from n/2+1 Hermitian bins."""
+    X = X.contiguous()
+    out = torch.empty(*X.shape[:-1], n, dtype=_REAL_OF[X.dtype], device=X.device)
+    RealFFT(n, _REAL_OF[X.dtype]).inverse(out, X)
+    return out
+
+
+def fft2(x: torch.Tensor, inverse: bool = False) -> torch.Tensor:
+    """2D transform over the last two axes (rows = x.shape[-2], cols = x.shape[-1])."""
+    x = x.contiguous()
+    H, W = x.shape[-2], x.shape[-1]
+    b = max(1, x.numel() // (H * W))
+    out = torch.empty_like(x)
+    FFT2D(W, H, x.dtype).transform(out, W, x, W, inverse, batch=b)
+    return out

@@ -1,0 +1,450 @@
+// gfft_device.h -- device-side building blocks of the gfx950 FFT engine.
+//
+// One kernel template, `tile_fft_kernel`, does every complex pass:
+//
+//   * a workgroup owns a TILE of M independent sequences of length L;
+//   * the tile is read from HBM with wide coalesced loads into LDS
+//     (INP = ROW: a sequence's elements are contiguous; INP = COL: the M
+//     sequences are contiguous, elements are `in_elem` apart);
+//   * log_R(L) self-sorting Stockham radix-R passes run LDS -> registers ->
+//     LDS (radix 2/4/8/16 butterflies in registers), so no bit reversal pass
+//     exists anywhere;
+//   * the last pass may multiply by the four-step twiddle W_Ntw^(c*k)
+//     (generated from FP64 tables, rounded once);
+//   * the tile is written back with coalesced stores (OUTP = ROW or COL).
+//
+// Batched 1D rows, the column pass of a four-step decomposition, the
+// transposing row pass, and the 2D row / column passes are all instances.
+// The inverse transform is the forward one with re/im swapped on the first
+// load and the last store: swap(DFT(swap(x))) = unscaled IDFT(x).
+//
+// The math replaces genFFT's radix-2 DIT stages (generic/fft_impl_generic.h:36-120,
+// x86/fft_float_impl_x86.inl:31-221): forward kernel exp(-2*pi*i*j*k/N),
+// unscaled inverse, natural-order input and output.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gfft {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b)
+{
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// a * (-i)
+template <typename T>
+__device__ __forceinline__ cx<T> mul_mi(cx<T> a) { return {a.im, -a.re}; }
+template <typename T>
+__device__ __forceinline__ cx<T> swap_ri(cx<T> a) { return {a.im, a.re}; }
+
+// ---------------------------------------------------------------------------
+// In-register DFTs of size 2/4/8/16, forward sign, natural order in and out.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void dft2(cx<T> *v)
+{
+    cx<T> a = v[0];
+    v[0] = a + v[1];
+    v[1] = a - v[1];
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cx<T> &x0, cx<T> &x1, cx<T> &x2, cx<T> &x3)
+{
+    cx<T> t0 = x0 + x2, t1 = x0 - x2, t2 = x1 + x3, t3 = mul_mi(x1 - x3);
+    x0 = t0 + t2;
+    x2 = t0 - t2;
+    x1 = t1 + t3;
+    x3 = t1 - t3;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cx<T> *v) { dft4(v[0], v[1], v[2], v[3]); }
+
+template <typename T>
+__device__ __forceinline__ void dft8(cx<T> *v)
+{
+    const T r = (T)0.70710678118654752440084436210484903928L;
+    // decimation in frequency: a_n = x_n + x_{n+4}, b_n = (x_n - x_{n+4}) W8^n
+    cx<T> a0 = v[0] + v[4], a1 = v[1] + v[5], a2 = v[2] + v[6], a3 = v[3] + v[7];
+    cx<T> b0 = v[0] - v[4], b1 = v[1] - v[5], b2 = v[2] - v[6], b3 = v[3] - v[7];
+    b1 = {r * (b1.re + b1.im), r * (b1.im - b1.re)};
+    b2 = mul_mi(b2);
+    b3 = {r * (b3.im - b3.re), -r * (b3.re + b3.im)};
+    dft4(a0, a1, a2, a3);
+    dft4(b0, b1, b2, b3);
+    v[0] = a0; v[2] = a1; v[4] = a2; v[6] = a3;
+    v[1] = b0; v[3] = b1; v[5] = b2; v[7] = b3;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft16(cx<T> *v)
+{
+    const T c1 = (T)0.92387953251128675612818318939678828682L;  // cos(pi/8)
+    const T s1 = (T)0.38268343236508977172845998403039886676L;  // sin(pi/8)
+    const T r = (T)0.70710678118654752440084436210484903928L;
+    // n = 4*n1 + n2, k = k1 + 4*k2
+    cx<T> a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        a[n2][0] = v[n2];
+        a[n2][1] = v[n2 + 4];
+        a[n2][2] = v[n2 + 8];
+        a[n2][3] = v[n2 + 12];
+        dft4(a[n2]);
+    }
+    // a[n2][k1] *= W16^(n2*k1)
+    a[1][1] = cmul(a[1][1], cx<T>{c1, -s1});
+    a[1][2] = {r * (a[1][2].re + a[1][2].im), r * (a[1][2].im - a[1][2].re)};
+    a[1][3] = cmul(a[1][3], cx<T>{s1, -c1});
+    a[2][1] = {r * (a[2][1].re + a[2][1].im), r * (a[2][1].im - a[2][1].re)};
+    a[2][2] = mul_mi(a[2][2]);
+    a[2][3] = {r * (a[2][3].im - a[2][3].re), -r * (a[2][3].re + a[2][3].im)};
+    a[3][1] = cmul(a[3][1], cx<T>{s1, -c1});
+    a[3][2] = {r * (a[3][2].im - a[3][2].re), -r * (a[3][2].re + a[3][2].im)};
+    a[3][3] = cmul(a[3][3], cx<T>{-c1, s1});
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        cx<T> x0 = a[0][k1], x1 = a[1][k1], x2 = a[2][k1], x3 = a[3][k1];
+        dft4(x0, x1, x2, x3);
+        v[k1] = x0;
+        v[k1 + 4] = x1;
+        v[k1 + 8] = x2;
+        v[k1 + 12] = x3;
+    }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void dft(cx<T> *v)
+{
+    if constexpr (R == 2) dft2(v);
+    else if constexpr (R == 4) dft4(v);
+    else if constexpr (R == 8) dft8(v);
+    else if constexpr (R == 16) dft16(v);
+    else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
+}
+
+// ---------------------------------------------------------------------------
+// Tile geometry and kernel arguments
+// ---------------------------------------------------------------------------
+enum : int { PAT_ROW = 0, PAT_COL = 1 };
+enum : int { LAY_SEQ = 0, LAY_ELEM = 1 };
+enum : int { TW_NONE = 0, TW_INNER = 1, TW_OUTER = 2 };
+
+struct TileArgs {
+    const void *in;
+    void *out;
+    // sequence (o2, o1, c) starts at o2*in_outer2 + o1*in_outer + c*in_inner;
+    // its element n sits in_elem further on (strides in complex elements)
+    int64_t in_outer2, in_outer, in_inner, in_elem;
+    int64_t out_outer2, out_outer, out_inner, out_elem;
+    int32_t n_inner;      // sequences along the inner index c
+    int32_t tiles_inner;  // ceil(n_inner / M)
+    int32_t n_outer1;     // range of o1
+    int32_t n_tiles;      // tiles_inner * n_outer1 * n_outer2
+    int32_t swap_in, swap_out;
+    int32_t vec_in, vec_out;  // 16-byte vector access allowed (alignment proven on host)
+    int32_t tw_mode;      // TW_NONE / TW_INNER (c * k) / TW_OUTER (o1 * k)
+    uint32_t tw_mask;     // Ntw - 1
+    int32_t tw_lo_bits;   // two-level split of the exponent
+    const void *pass_tw;  // Stockham pass twiddles, complex T
+    const double *tw_lo;  // complex double [1 << tw_lo_bits]: W_Ntw^m
+    const double *tw_hi;  // complex double [Ntw >> tw_lo_bits]: W_Ntw^(m << tw_lo_bits)
+};
+
+// FP64 twiddle W_Ntw^m from the two-level table.
+__device__ __forceinline__ cx<double> tw_fp64(const TileArgs &a, uint32_t m)
+{
+    m &= a.tw_mask;
+    const cx<double> *lo = reinterpret_cast<const cx<double> *>(a.tw_lo);
+    const cx<double> *hi = reinterpret_cast<const cx<double> *>(a.tw_hi);
+    cx<double> l = lo[m & ((1u << a.tw_lo_bits) - 1u)];
+    cx<double> h = hi[m >> a.tw_lo_bits];
+    return cmul(h, l);
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> apply_tw(cx<T> v, cx<double> w)
+{
+    if constexpr (sizeof(T) == 8) {
+        return cmul(v, w);
+    } else {
+        cx<T> wt = {(T)w.re, (T)w.im};
+        return cmul(v, wt);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One Stockham pass of radix R on every sequence of the tile, in LDS.
+//   NS = product of the radices of the previous passes.
+//   butterfly (s, j), j < L/R: inputs j + r*L/R, twiddle W_{NS*R}^(r*(j%NS)),
+//   outputs (j - j%NS)*R + j%NS + r*NS.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int M, int NT, class G, bool FASTJ, int NS, int R, bool LAST>
+__device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restrict__ tw, int tid,
+                                              const TileArgs &a, int s0, int o)
+{
+    constexpr int LR = L / R;
+    constexpr int BF = M * LR;
+    constexpr int PER = (BF + NT - 1) / NT;
+    cx<T> v[PER][R];
+    int sj[PER], jj[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int b = tid + i * NT;
+        int s, j;
+        if constexpr (FASTJ) {
+            j = b % LR;
+            s = b / LR;
+        } else {
+            s = b % M;
+            j = b / M;
+        }
+        sj[i] = s;
+        jj[i] = j;
+        if (BF % NT == 0 || b < BF) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[i][r] = lds[G::off(s, j + r * LR)];
+        }
+    }
+    cx<T> w[PER][R > 1 ? R - 1 : 1];
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int m = jj[i] & (NS - 1);
+#pragma unroll
+            for (int r = 1; r < R; r++) w[i][r - 1] = tw[m * (R - 1) + (r - 1)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int b = tid + i * NT;
+        if (BF % NT == 0 || b < BF) {
+            const int s = sj[i], j = jj[i];
+            const int m = j & (NS - 1);
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], w[i][r - 1]);
+            }
+            dft<R>(v[i]);
+            if constexpr (LAST) {
+                // four-step twiddle W_Ntw^(c*k), k = j + r*L/R (here NS*R == L)
+                if (a.tw_mode != TW_NONE) {
+                    const uint32_t c = a.tw_mode == TW_INNER ? (uint32_t)(s0 + s) : (uint32_t)o;
+                    if constexpr (sizeof(T) == 4) {
+                        // FP64 geometric progression, rounded once per element
+                        cx<double> wk = tw_fp64(a, c * (uint32_t)j);
+                        const cx<double> ws = tw_fp64(a, c * (uint32_t)LR);
+                        v[i][0] = apply_tw(v[i][0], wk);
+#pragma unroll
+                        for (int r = 1; r < R; r++) {
+                            wk = cmul(wk, ws);
+                            v[i][r] = apply_tw(v[i][r], wk);
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < R; r++)
+                            v[i][r] = apply_tw(v[i][r], tw_fp64(a, c * (uint32_t)(j + r * LR)));
+                    }
+                }
+            }
+            const int base = (j - m) * R + m;
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[G::off(s, base + r * NS)] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int L, int M, int NT, class G, bool FASTJ, int NS, int R, int... RS>
+__device__ __forceinline__ void stockham_passes(cx<T> *lds, const cx<T> *__restrict__ tw, int tid,
+                                                const TileArgs &a, int s0, int o)
+{
+    constexpr bool last = sizeof...(RS) == 0;
+    stockham_pass<T, L, M, NT, G, FASTJ, NS, R, last>(lds, tw, tid, a, s0, o);
+    if constexpr (!last) {
+        constexpr int used = NS > 1 ? NS * (R - 1) : 0;
+        stockham_passes<T, L, M, NT, G, FASTJ, NS * R, RS...>(lds, tw + used, tid, a, s0, o);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// HBM <-> LDS tile movement
+// ---------------------------------------------------------------------------
+template <typename T, int L, int M, int NT, class G, int INP>
+__device__ __forceinline__ void load_tile(cx<T> *lds, const TileArgs &a, int o2, int o, int s0,
+                                          int nvalid, int tid)
+{
+    constexpr int TOT = M * L;
+    const bool sw = a.swap_in != 0;
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(a.in) + (int64_t)o * a.in_outer +
+                      (int64_t)o2 * a.in_outer2;
+    if constexpr (INP == PAT_ROW) in += (int64_t)s0 * a.in_inner;
+    else in += s0;
+
+    if constexpr (sizeof(T) == 4 && TOT % (2 * NT) == 0 && (INP == PAT_ROW ? L : M) % 2 == 0) {
+        if (a.vec_in && nvalid == M) {
+#pragma unroll
+            for (int it = 0; it < TOT / (2 * NT); it++) {
+                const int e = (tid + it * NT) * 2;
+                int s, n;
+                const float4 *p;
+                if constexpr (INP == PAT_ROW) {
+                    s = e / L;
+                    n = e % L;
+                    p = reinterpret_cast<const float4 *>(in + (int64_t)s * a.in_inner + n);
+                    float4 q = *p;
+                    cx<T> x0 = {q.x, q.y}, x1 = {q.z, q.w};
+                    lds[G::off(s, n)] = sw ? swap_ri(x0) : x0;
+                    lds[G::off(s, n + 1)] = sw ? swap_ri(x1) : x1;
+                } else {
+                    n = e / M;
+                    s = e % M;
+                    p = reinterpret_cast<const float4 *>(in + (int64_t)n * a.in_elem + s);
+                    float4 q = *p;
+                    cx<T> x0 = {q.x, q.y}, x1 = {q.z, q.w};
+                    lds[G::off(s, n)] = sw ? swap_ri(x0) : x0;
+                    lds[G::off(s + 1, n)] = sw ? swap_ri(x1) : x1;
+                }
+            }
+            return;
+        }
+    }
+#pragma unroll 4
+    for (int it = 0; it < (TOT + NT - 1) / NT; it++) {
+        const int e = tid + it * NT;
+        if (TOT % NT == 0 || e < TOT) {
+            int s, n;
+            if constexpr (INP == PAT_ROW) {
+                s = e / L;
+                n = e % L;
+            } else {
+                n = e / M;
+                s = e % M;
+            }
+            if (s < nvalid) {
+                const cx<T> *p = INP == PAT_ROW ? in + (int64_t)s * a.in_inner + n
+                                                : in + (int64_t)n * a.in_elem + s;
+                cx<T> x = *p;
+                lds[G::off(s, n)] = sw ? swap_ri(x) : x;
+            }
+        }
+    }
+}
+
+template <typename T, int L, int M, int NT, class G, int OUTP>
+__device__ __forceinline__ void store_tile(const cx<T> *lds, const TileArgs &a, int o2, int o,
+                                           int s0, int nvalid, int tid)
+{
+    constexpr int TOT = M * L;
+    const bool sw = a.swap_out != 0;
+    cx<T> *out = reinterpret_cast<cx<T> *>(a.out) + (int64_t)o * a.out_outer +
+                 (int64_t)o2 * a.out_outer2;
+    if constexpr (OUTP == PAT_ROW) out += (int64_t)s0 * a.out_inner;
+    else out += s0;
+
+    if constexpr (sizeof(T) == 4 && TOT % (2 * NT) == 0 && (OUTP == PAT_ROW ? L : M) % 2 == 0) {
+        if (a.vec_out && nvalid == M) {
+#pragma unroll
+            for (int it = 0; it < TOT / (2 * NT); it++) {
+                const int e = (tid + it * NT) * 2;
+                int s, k;
+                cx<T> x0, x1;
+                float4 *p;
+                if constexpr (OUTP == PAT_ROW) {
+                    s = e / L;
+                    k = e % L;
+                    x0 = lds[G::off(s, k)];
+                    x1 = lds[G::off(s, k + 1)];
+                    p = reinterpret_cast<float4 *>(out + (int64_t)s * a.out_inner + k);
+                } else {
+                    k = e / M;
+                    s = e % M;
+                    x0 = lds[G::off(s, k)];
+                    x1 = lds[G::off(s + 1, k)];
+                    p = reinterpret_cast<float4 *>(out + (int64_t)k * a.out_elem + s);
+                }
+                if (sw) {
+                    x0 = swap_ri(x0);
+                    x1 = swap_ri(x1);
+                }
+                float4 q = {x0.re, x0.im, x1.re, x1.im};
+                *p = q;
+            }
+            return;
+        }
+    }
+#pragma unroll 4
+    for (int it = 0; it < (TOT + NT - 1) / NT; it++) {
+        const int e = tid + it * NT;
+        if (TOT % NT == 0 || e < TOT) {
+            int s, k;
+            if constexpr (OUTP == PAT_ROW) {
+                s = e / L;
+                k = e % L;
+            } else {
+                k = e / M;
+                s = e % M;
+            }
+            if (s < nvalid) {
+                cx<T> x = lds[G::off(s, k)];
+                cx<T> *p = OUTP == PAT_ROW ? out + (int64_t)s * a.out_inner + k
+                                           : out + (int64_t)k * a.out_elem + s;
+                *p = sw ? swap_ri(x) : x;
+            }
+        }
+    }
+}
+
+// LDS geometry of a tile: element n of sequence s lives at G::off(s, n).
+//   LAY_SEQ : sequence-major.  One pad element after every 16 elements of a
+//             sequence breaks the stride-R conflicts of the Stockham writes;
+//             the sequence pitch is odd so walks across sequences (COL side
+//             of a transposing pass) hit distinct banks.
+//   LAY_ELEM: element-major (the M sequences of one element index adjacent),
+//             pitch padded by one when an HBM side walks along a sequence.
+template <int L, int M, int LAYOUT, int INP, int OUTP>
+struct TileGeom {
+    static constexpr int LP = L + (L >> 4);                       // padded sequence length
+    static constexpr int SS = LAYOUT == LAY_SEQ ? (LP | 1) : 1;   // sequence pitch
+    static constexpr int MS = M + ((INP == PAT_ROW || OUTP == PAT_ROW) ? 1 : 0);
+    static constexpr int ELEMS = LAYOUT == LAY_SEQ ? M * SS : L * MS;
+    static __device__ __forceinline__ int off(int s, int n)
+    {
+        if constexpr (LAYOUT == LAY_SEQ) return s * SS + n + (n >> 4);
+        else return n * MS + s;
+    }
+};
+
+template <typename T, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
+__global__ void __launch_bounds__(NT) tile_fft_kernel(TileArgs a)
+{
+    using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
+    __shared__ cx<T> lds[G::ELEMS];
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int t2 = tile / a.tiles_inner;
+    const int ti = tile - t2 * a.tiles_inner;
+    const int o2 = t2 / a.n_outer1;
+    const int o = t2 - o2 * a.n_outer1;
+    const int s0 = ti * M;
+    const int nvalid = min(M, a.n_inner - s0);
+    load_tile<T, L, M, NT, G, INP>(lds, a, o2, o, s0, nvalid, tid);
+    __syncthreads();
+    stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, 1, RS...>(
+        lds, reinterpret_cast<const cx<T> *>(a.pass_tw), tid, a, s0, o);
+    store_tile<T, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, tid);
+}
+
+}  // namespace gfft

@@ -1,0 +1,68 @@
+// gfft_misc.h -- argument blocks and launchers of the element-wise kernels
+// (kern_misc.hip).  Strides and distances are in elements of the buffer's own
+// type (complex, or real where a field says so).
+#pragma once
+#include "gfft_device.h"
+
+namespace gfft {
+
+// Two-level FP64 table of W_N^m = exp(-2*pi*i*m/N):
+//   W_N^m = hi[m >> lo_bits] * lo[m & ((1 << lo_bits) - 1)]
+struct TwTable {
+    const double *lo;
+    const double *hi;
+    uint64_t mask;  // N - 1
+    int32_t lo_bits;
+    int32_t pad;
+};
+
+struct UnpackArgs {
+    const void *in;
+    void *out;
+    int64_t n;         // real transform length N
+    int64_t batch;
+    int64_t in_dist;   // complex elements
+    int64_t out_dist;  // complex elements
+    int32_t half;      // write N/2+1 bins only
+    int32_t pad;
+    TwTable tw;        // W_N
+};
+
+struct SeparateArgs {
+    const void *in;
+    void *out1;
+    void *out2;
+    int64_t n, batch, dist;
+};
+
+struct InterleaveArgs {
+    const void *in1;
+    const void *in2;
+    void *out;
+    int64_t n, batch;
+};
+
+struct HFillArgs {
+    void *out;
+    int64_t M, N;      // columns (width), rows (height)
+    int64_t vcols;     // columns already computed (M/2 + 1)
+    int64_t stride;    // row pitch, complex elements
+    int64_t dist;      // batch distance, complex elements
+    int64_t batch;
+};
+
+struct CopyArgs {
+    const void *in;
+    void *out;
+    int64_t batch, in_dist, out_dist;
+    int32_t real_in, real_out;
+};
+
+hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s);
+hipError_t launch_pack(int prec, const UnpackArgs &a, hipStream_t s);
+hipError_t launch_separate(int prec, const SeparateArgs &a, hipStream_t s);
+hipError_t launch_interleave(int prec, const InterleaveArgs &a, hipStream_t s);
+hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s);
+hipError_t launch_copy(int prec, const CopyArgs &a, hipStream_t s);
+
+}  // namespace gfft

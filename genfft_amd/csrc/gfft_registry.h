@@ -1,0 +1,52 @@
+// gfft_registry.h -- the table of compiled tile-kernel shapes.
+//
+// Every instantiation of gfft::tile_fft_kernel is described by one entry;
+// the planner (plan.cpp) picks entries by (precision, L, input pattern,
+// output pattern) and launches them through the function pointer.
+#pragma once
+#include "gfft_device.h"
+
+namespace gfft {
+
+struct KernelEntry {
+    int prec;      // 0 = complex64, 1 = complex128
+    int L;         // sequence length
+    int M;         // sequences per tile
+    int NT;        // threads per workgroup
+    int layout;    // LAY_SEQ / LAY_ELEM
+    int inp, outp; // PAT_ROW / PAT_COL
+    int nrad;
+    int rad[8];
+    int lds_bytes;
+    void (*fn)(TileArgs);
+};
+
+template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
+constexpr KernelEntry make_entry()
+{
+    static_assert(((RS * ...) == L), "radices must multiply to L");
+    static_assert((M * L) % NT == 0, "tile must split evenly over threads");
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole wavefronts, at most 1024 threads");
+    using G = TileGeom<L, M, LAY, INP, OUTP>;
+    static_assert(G::ELEMS * sizeof(cx<T>) <= 160 * 1024, "tile exceeds gfx950 LDS");
+    KernelEntry e{};
+    e.prec = sizeof(T) == 8 ? 1 : 0;
+    e.L = L;
+    e.M = M;
+    e.NT = NT;
+    e.layout = LAY;
+    e.inp = INP;
+    e.outp = OUTP;
+    e.nrad = sizeof...(RS);
+    const int r[] = {RS...};
+    for (int i = 0; i < (int)sizeof...(RS); i++) e.rad[i] = r[i];
+    e.lds_bytes = (int)(G::ELEMS * sizeof(cx<T>));
+    e.fn = &tile_fft_kernel<T, L, M, NT, LAY, INP, OUTP, RS...>;
+    return e;
+}
+
+// Defined in kern_tile_f32.hip / kern_tile_f64.hip.
+const KernelEntry *tile_kernels_f32(int *count);
+const KernelEntry *tile_kernels_f64(int *count);
+
+}  // namespace gfft

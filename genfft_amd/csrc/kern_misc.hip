@@ -1,0 +1,246 @@
+// kern_misc.hip -- element-wise kernels around the Stockham tile passes:
+//
+//   unpack      real-FFT post-pass, the GPU form of adjust_DIT_impl
+//               (generic/fft_dit_impl_generic.inl:27-61): F from the n/2-point
+//               FFT Z of the real signal packed as complex.
+//   pack        its inverse (half spectrum F -> Z), the c2r pre-pass; the
+//               reference has no c2r (include/genFFT/FFTReal.h:127-130).
+//   separate    separate_2x_real_FFT (include/genFFT/FFTReal.h:36-66).
+//   interleave  transform_interleave's packing (include/genFFT/fft.h:100-105).
+//   hfill       RealFFT2D's Hermitian column fill (include/genFFT/FFTReal.h:90-103).
+//   copy        the n = 1 transform (identity on each element).
+//
+// All of them are grid-stride loops over (pair, batch) with 64-bit indices
+// and read each input element once.
+#include "gfft_misc.h"
+
+namespace gfft {
+
+namespace {
+
+// W_N^m (forward sign) from the two-level FP64 table, then i*W = (sin, cos).
+__device__ __forceinline__ cx<double> twl(const TwTable &t, uint64_t m)
+{
+    m &= t.mask;
+    const cx<double> *lo = reinterpret_cast<const cx<double> *>(t.lo);
+    const cx<double> *hi = reinterpret_cast<const cx<double> *>(t.hi);
+    return cmul(hi[m >> t.lo_bits], lo[m & ((1ull << t.lo_bits) - 1ull)]);
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.re, -a.im}; }
+
+template <typename T>
+__device__ __forceinline__ cx<T> scale(cx<T> a, T s) { return {a.re * s, a.im * s}; }
+
+template <typename T>
+__global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a)
+{
+    using C = cx<T>;
+    const C *Z = reinterpret_cast<const C *>(a.in);
+    C *F = reinterpret_cast<C *>(a.out);
+    const int64_t N = a.n, halfN = N >> 1, quarter = N >> 2;
+    const int64_t pairs = quarter + 1;  // p = 0 .. N/4
+    const int64_t total = pairs * a.batch;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = g / pairs, p = g - b * pairs;
+        const C *z = Z + b * a.in_dist;
+        C *f = F + b * a.out_dist;
+        if (p == 0) {
+            const C z0 = z[0];
+            f[0] = {z0.re + z0.im, (T)0};
+            if (N >= 2) f[halfN] = {z0.re - z0.im, (T)0};
+            continue;
+        }
+        if (p == quarter) {  // N >= 4: F[N/4] = conj(Z[N/4])
+            const C zq = z[quarter];
+            f[quarter] = conj(zq);
+            if (!a.half) f[N - quarter] = zq;
+            continue;
+        }
+        const int64_t i = p, j = halfN - p;
+        const C zi = z[i], zj = z[j];
+        const C E = scale(zi + conj(zj), (T)0.5);
+        const C O = scale(zi - conj(zj), (T)0.5);
+        const cx<double> w = twl(a.tw, (uint64_t)i);
+        const C t = {(T)(-w.im), (T)w.re};  // i * W_N^i = (sin, cos)
+        const C to = cmul(t, O);
+        const C fi = E - to;
+        const C fj = conj(E + to);
+        f[i] = fi;
+        f[j] = fj;
+        if (!a.half) {
+            f[N - i] = conj(fi);
+            f[N - j] = conj(fj);
+        }
+    }
+}
+
+// Z[k] = (F[k] + conj F[L-k]) + i W_N^-k (F[k] - conj F[L-k]),  L = N/2.
+// The unscaled inverse L-point FFT of Z, read as N reals, is N * x.
+template <typename T>
+__global__ void __launch_bounds__(256) pack_kernel(UnpackArgs a)
+{
+    using C = cx<T>;
+    const C *F = reinterpret_cast<const C *>(a.in);
+    C *Z = reinterpret_cast<C *>(a.out);
+    const int64_t N = a.n, L = N >> 1;
+    const int64_t total = L * a.batch;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = g / L, k = g - b * L;
+        const C *f = F + b * a.in_dist;
+        const C fk = f[k], fm = f[L - k];
+        const C s = fk + conj(fm);
+        const C d = fk - conj(fm);
+        const cx<double> w = twl(a.tw, (uint64_t)k);
+        const C wc = {(T)w.re, (T)(-w.im)};  // W_N^-k
+        const C q = cmul(wc, d);
+        Z[b * a.out_dist + k] = {s.re - q.im, s.im + q.re};  // s + i q
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) separate_kernel(SeparateArgs a)
+{
+    using C = cx<T>;
+    const C *Z = reinterpret_cast<const C *>(a.in);
+    C *X = reinterpret_cast<C *>(a.out1);
+    C *Y = reinterpret_cast<C *>(a.out2);
+    const int64_t N = a.n, pairs = N / 2 + 1;
+    const int64_t total = pairs * a.batch;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = g / pairs, i = g - b * pairs;
+        const C *z = Z + b * a.dist;
+        C *x = X + b * a.dist;
+        C *y = Y + b * a.dist;
+        if (i == 0) {
+            const C z0 = z[0];
+            x[0] = {z0.re, (T)0};
+            y[0] = {z0.im, (T)0};
+            continue;
+        }
+        const int64_t k = N - i;
+        const C zi = z[i], zk = z[k];
+        const C xv = {(zi.re + zk.re) * (T)0.5, (zi.im - zk.im) * (T)0.5};
+        const C yv = {(zk.im + zi.im) * (T)0.5, (zk.re - zi.re) * (T)0.5};
+        x[i] = xv;
+        y[i] = yv;
+        x[k] = conj(xv);
+        y[k] = conj(yv);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) interleave_kernel(InterleaveArgs a)
+{
+    using C = cx<T>;
+    const T *x = reinterpret_cast<const T *>(a.in1);
+    const T *y = reinterpret_cast<const T *>(a.in2);
+    C *z = reinterpret_cast<C *>(a.out);
+    const int64_t total = a.n * a.batch;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x)
+        z[g] = {x[g], y[g]};
+}
+
+// out[k][j] = conj(out[i][M - j]) for j in [vcols, M), k = (N - i) mod N.
+template <typename T>
+__global__ void __launch_bounds__(256) hfill_kernel(HFillArgs a)
+{
+    using C = cx<T>;
+    C *out = reinterpret_cast<C *>(a.out);
+    const int64_t span = a.M - a.vcols;
+    const int64_t per = span * a.N;
+    const int64_t total = per * a.batch;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = g / per, r = g - b * per;
+        const int64_t i = r / span, j = a.vcols + (r - i * span);
+        const int64_t k = i ? a.N - i : 0;
+        C *base = out + b * a.dist;
+        base[k * a.stride + j] = conj(base[i * a.stride + (a.M - j)]);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) copy_kernel(CopyArgs a)
+{
+    using C = cx<T>;
+    const C *in = reinterpret_cast<const C *>(a.in);
+    C *out = reinterpret_cast<C *>(a.out);
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < a.batch;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        if (a.real_in) {
+            const T *r = reinterpret_cast<const T *>(a.in);
+            out[g * a.out_dist] = {r[g * a.in_dist], (T)0};
+        } else if (a.real_out) {
+            T *r = reinterpret_cast<T *>(a.out);
+            r[g * a.out_dist] = in[g * a.in_dist].re;
+        } else {
+            out[g * a.out_dist] = in[g * a.in_dist];
+        }
+    }
+}
+
+unsigned grid_for(int64_t work)
+{
+    int64_t blocks = (work + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 65536) blocks = 65536;
+    return (unsigned)blocks;
+}
+
+}  // namespace
+
+hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s)
+{
+    const unsigned g = grid_for(((a.n >> 2) + 1) * a.batch);
+    if (prec) hipLaunchKernelGGL(unpack_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(unpack_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(int prec, const UnpackArgs &a, hipStream_t s)
+{
+    const unsigned g = grid_for((a.n >> 1) * a.batch);
+    if (prec) hipLaunchKernelGGL(pack_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(pack_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_separate(int prec, const SeparateArgs &a, hipStream_t s)
+{
+    const unsigned g = grid_for((a.n / 2 + 1) * a.batch);
+    if (prec) hipLaunchKernelGGL(separate_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(separate_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_interleave(int prec, const InterleaveArgs &a, hipStream_t s)
+{
+    const unsigned g = grid_for(a.n * a.batch);
+    if (prec) hipLaunchKernelGGL(interleave_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(interleave_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s)
+{
+    const unsigned g = grid_for((a.M - a.vcols) * a.N * a.batch);
+    if (prec) hipLaunchKernelGGL(hfill_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(hfill_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(int prec, const CopyArgs &a, hipStream_t s)
+{
+    const unsigned g = grid_for(a.batch);
+    if (prec) hipLaunchKernelGGL(copy_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(copy_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gfft

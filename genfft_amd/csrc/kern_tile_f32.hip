@@ -1,0 +1,67 @@
+// kern_tile_f32.hip -- complex64 tile-kernel shapes.
+//
+// Rows (PAT_ROW -> PAT_ROW, sequence-major LDS): batched 1D transforms and
+// the 2D row pass; a tile holds 4096 complex (32 KB) up to L = 4096, then a
+// single sequence up to L = 16384 (136 KB of gfx950's 160 KB LDS).
+// Columns (PAT_COL -> PAT_COL, element-major LDS): the first pass of a
+// four-step transform and the 2D column pass; M adjacent columns give
+// >= 64-byte contiguous row segments.
+// Transposing rows (PAT_ROW -> PAT_COL): the second four-step pass, writing
+// element k of row k1 to k1 + N1*k, M rows wide.
+// Every shape keeps 16 elements per thread, so radix-16 butterflies live
+// entirely in registers.
+#include "gfft_registry.h"
+
+namespace gfft {
+namespace {
+using T = float;
+constexpr int R_ = PAT_ROW, C_ = PAT_COL, S_ = LAY_SEQ, E_ = LAY_ELEM;
+
+const KernelEntry kTable[] = {
+    // --- batched rows ---
+    make_entry<T, 2, 1024, 128, S_, R_, R_, 2>(),
+    make_entry<T, 4, 1024, 256, S_, R_, R_, 4>(),
+    make_entry<T, 8, 512, 256, S_, R_, R_, 8>(),
+    make_entry<T, 16, 256, 256, S_, R_, R_, 16>(),
+    make_entry<T, 32, 128, 256, S_, R_, R_, 8, 4>(),
+    make_entry<T, 64, 64, 256, S_, R_, R_, 8, 8>(),
+    make_entry<T, 128, 32, 256, S_, R_, R_, 16, 8>(),
+    make_entry<T, 256, 16, 256, S_, R_, R_, 16, 16>(),
+    make_entry<T, 512, 8, 256, S_, R_, R_, 8, 8, 8>(),
+    make_entry<T, 1024, 4, 256, S_, R_, R_, 16, 16, 4>(),
+    make_entry<T, 2048, 2, 256, S_, R_, R_, 16, 16, 8>(),
+    make_entry<T, 4096, 1, 256, S_, R_, R_, 16, 16, 16>(),
+    make_entry<T, 8192, 1, 512, S_, R_, R_, 16, 16, 8, 4>(),
+    make_entry<T, 16384, 1, 1024, S_, R_, R_, 16, 16, 16, 4>(),
+    // --- column tiles ---
+    make_entry<T, 2, 512, 64, E_, C_, C_, 2>(),
+    make_entry<T, 4, 256, 64, E_, C_, C_, 4>(),
+    make_entry<T, 8, 128, 64, E_, C_, C_, 8>(),
+    make_entry<T, 16, 64, 64, E_, C_, C_, 16>(),
+    make_entry<T, 32, 64, 128, E_, C_, C_, 8, 4>(),
+    make_entry<T, 64, 64, 256, E_, C_, C_, 8, 8>(),
+    make_entry<T, 128, 32, 256, E_, C_, C_, 16, 8>(),
+    make_entry<T, 256, 16, 256, E_, C_, C_, 16, 16>(),
+    make_entry<T, 512, 16, 512, E_, C_, C_, 8, 8, 8>(),
+    make_entry<T, 1024, 16, 1024, E_, C_, C_, 16, 16, 4>(),
+    make_entry<T, 1024, 8, 512, E_, C_, C_, 16, 16, 4>(),
+    make_entry<T, 2048, 8, 1024, E_, C_, C_, 16, 16, 8>(),
+    // --- transposing rows ---
+    make_entry<T, 64, 64, 256, S_, R_, C_, 8, 8>(),
+    make_entry<T, 128, 32, 256, S_, R_, C_, 16, 8>(),
+    make_entry<T, 256, 16, 256, S_, R_, C_, 16, 16>(),
+    make_entry<T, 512, 16, 512, S_, R_, C_, 8, 8, 8>(),
+    make_entry<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
+    make_entry<T, 1024, 8, 512, S_, R_, C_, 16, 16, 4>(),
+    make_entry<T, 2048, 8, 1024, S_, R_, C_, 16, 16, 8>(),
+    make_entry<T, 4096, 4, 1024, S_, R_, C_, 16, 16, 16>(),
+};
+}  // namespace
+
+const KernelEntry *tile_kernels_f32(int *count)
+{
+    *count = (int)(sizeof(kTable) / sizeof(kTable[0]));
+    return kTable;
+}
+
+}  // namespace gfft

@@ -1,0 +1,1011 @@
+// plan.cpp -- planner and C ABI of the gfx950 FFT engine (include/genfft_amd.h).
+//
+// A plan is a fixed list of kernel launches over three buffers (the caller's
+// input and output and a plan-owned scratch), built once at creation:
+//
+//   C2C, n <= 16384 (c64) / 8192 (c128)  one batched row pass, one HBM round trip
+//   C2C, larger n = N1 * N2                column pass (N1-point FFTs down N2 columns,
+//                                          times W_n^(n2*k1)) into scratch, then a
+//                                          transposing row pass (N2-point FFTs written
+//                                          to k1 + N1*k2); the batch is processed in
+//                                          chunks so the scratch ring stays small
+//   R2C                                    C2C of n/2 on the reals viewed as complex,
+//                                          then the unpack pass
+//   C2R                                    pack pass, then an inverse C2C of n/2
+//   DIT                                    the unpack pass alone
+//   COLS                                   column tiles (two sub-passes when long)
+//   C2C_2D                                 row pass, then column pass(es)
+//   R2C_2D                                 row R2C on every row, column C2C on the
+//                                          first width/2+1 columns, Hermitian fill
+//
+// Twiddles: every table is computed in long double with exact integer
+// reduction of the angle and rounded once to the working precision; the
+// four-step and real-FFT twiddles come from two-level FP64 tables multiplied
+// in FP64 on the device, then rounded once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/genfft_amd.h"
+#include "gfft_misc.h"
+#include "gfft_registry.h"
+
+namespace gfft {
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(GFFT_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));        \
+    } while (0)
+
+bool is_pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
+int ilog2(int64_t n)
+{
+    int l = 0;
+    while (((int64_t)1 << l) < n) l++;
+    return l;
+}
+
+// ---------------------------------------------------------------------------
+// Twiddles: exp(-2*pi*i*m/n) in long double, m reduced mod n and folded into
+// the first octant before cosl / sinl.
+// ---------------------------------------------------------------------------
+void twiddle_ld(int64_t m, int64_t n, long double *c, long double *s)
+{
+    static const long double PI = 3.141592653589793238462643383279502884197L;
+    m %= n;
+    if (m < 0) m += n;
+    const int64_t a = 8 * m, q = a / n, r = a - q * n;
+    const int64_t rr = (q & 1) ? (n - r) : r;
+    const long double x = (2.0L * PI * (long double)rr) / (8.0L * (long double)n);
+    const long double cx = cosl(x), sx = sinl(x);
+    long double C, S;
+    switch (q) {
+    case 0: C = cx; S = sx; break;
+    case 1: C = sx; S = cx; break;
+    case 2: C = -sx; S = cx; break;
+    case 3: C = -cx; S = sx; break;
+    case 4: C = -cx; S = -sx; break;
+    case 5: C = -sx; S = -cx; break;
+    case 6: C = sx; S = -cx; break;
+    default: C = cx; S = -sx; break;
+    }
+    *c = C;
+    *s = -S;
+}
+
+struct DeviceTables {
+    std::mutex mu;
+    // (device, prec, L, radices) -> pass twiddles
+    std::map<std::tuple<int, int, int, std::vector<int>>, void *> pass;
+    // (device, N) -> two-level FP64 table {lo, hi}
+    std::map<std::pair<int, int64_t>, std::pair<void *, void *>> two_level;
+};
+
+DeviceTables &tables()
+{
+    static DeviceTables *t = new DeviceTables();  // lives for the process
+    return *t;
+}
+
+int upload(void **dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMalloc(dst, bytes));
+    HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return GFFT_OK;
+}
+
+// Stockham pass twiddles of one tile shape: for each pass with NS > 1,
+// entries [m*(R-1) + r-1] = W_{NS*R}^(r*m), m < NS, r = 1..R-1.
+int get_pass_twiddles(int dev, const KernelEntry &k, const void **out, bool dry)
+{
+    if (dry) {
+        *out = nullptr;
+        return GFFT_OK;
+    }
+    auto &T = tables();
+    std::lock_guard<std::mutex> lk(T.mu);
+    std::vector<int> rad(k.rad, k.rad + k.nrad);
+    auto key = std::make_tuple(dev, k.prec, k.L, rad);
+    auto it = T.pass.find(key);
+    if (it != T.pass.end()) {
+        *out = it->second;
+        return GFFT_OK;
+    }
+    std::vector<long double> re, im;
+    int64_t ns = 1;
+    for (int p = 0; p < k.nrad; p++) {
+        const int R = k.rad[p];
+        if (ns > 1)
+            for (int64_t m = 0; m < ns; m++)
+                for (int r = 1; r < R; r++) {
+                    long double c, s;
+                    twiddle_ld((int64_t)r * m, ns * R, &c, &s);
+                    re.push_back(c);
+                    im.push_back(s);
+                }
+        ns *= R;
+    }
+    if (re.empty()) {  // single-pass shapes read no twiddles; keep a valid pointer
+        re.push_back(1);
+        im.push_back(0);
+    }
+    void *d = nullptr;
+    int st;
+    if (k.prec) {
+        std::vector<double> h(2 * re.size());
+        for (size_t i = 0; i < re.size(); i++) h[2 * i] = (double)re[i], h[2 * i + 1] = (double)im[i];
+        st = upload(&d, h.data(), h.size() * sizeof(double));
+    } else {
+        std::vector<float> h(2 * re.size());
+        for (size_t i = 0; i < re.size(); i++) h[2 * i] = (float)re[i], h[2 * i + 1] = (float)im[i];
+        st = upload(&d, h.data(), h.size() * sizeof(float));
+    }
+    if (st) return st;
+    T.pass[key] = d;
+    *out = d;
+    return GFFT_OK;
+}
+
+int get_two_level(int dev, int64_t N, TwTable *t, bool dry)
+{
+    auto &T = tables();
+    std::lock_guard<std::mutex> lk(T.mu);
+    const int lg = ilog2(N);
+    const int lo_bits = (lg + 1) / 2;
+    t->mask = (uint64_t)(N - 1);
+    t->lo_bits = lo_bits;
+    t->pad = 0;
+    if (dry) {
+        t->lo = t->hi = nullptr;
+        return GFFT_OK;
+    }
+    auto key = std::make_pair(dev, N);
+    auto it = T.two_level.find(key);
+    if (it != T.two_level.end()) {
+        t->lo = (const double *)it->second.first;
+        t->hi = (const double *)it->second.second;
+        return GFFT_OK;
+    }
+    const int64_t nlo = (int64_t)1 << lo_bits, nhi = std::max<int64_t>(1, N >> lo_bits);
+    std::vector<double> lo(2 * nlo), hi(2 * nhi);
+    for (int64_t m = 0; m < nlo; m++) {
+        long double c, s;
+        twiddle_ld(m, N, &c, &s);
+        lo[2 * m] = (double)c;
+        lo[2 * m + 1] = (double)s;
+    }
+    for (int64_t h = 0; h < nhi; h++) {
+        long double c, s;
+        twiddle_ld(h << lo_bits, N, &c, &s);
+        hi[2 * h] = (double)c;
+        hi[2 * h + 1] = (double)s;
+    }
+    void *dlo = nullptr, *dhi = nullptr;
+    int st = upload(&dlo, lo.data(), lo.size() * sizeof(double));
+    if (st) return st;
+    st = upload(&dhi, hi.data(), hi.size() * sizeof(double));
+    if (st) return st;
+    T.two_level[key] = {dlo, dhi};
+    t->lo = (const double *)dlo;
+    t->hi = (const double *)dhi;
+    return GFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel selection
+// ---------------------------------------------------------------------------
+const KernelEntry *find_tile(int prec, int L, int inp, int outp, int prefer_m = 0)
+{
+    int cnt = 0;
+    const KernelEntry *tab = prec ? tile_kernels_f64(&cnt) : tile_kernels_f32(&cnt);
+    const KernelEntry *best = nullptr;
+    for (int i = 0; i < cnt; i++) {
+        const KernelEntry &e = tab[i];
+        if (e.L != L || e.inp != inp || e.outp != outp) continue;
+        if (prefer_m && e.M == prefer_m) return &e;
+        if (!best || e.M > best->M) best = &e;
+    }
+    return best;
+}
+
+int max_len(int prec, int inp, int outp)
+{
+    int cnt = 0, best = 0;
+    const KernelEntry *tab = prec ? tile_kernels_f64(&cnt) : tile_kernels_f32(&cnt);
+    for (int i = 0; i < cnt; i++)
+        if (tab[i].inp == inp && tab[i].outp == outp) best = std::max(best, tab[i].L);
+    return best;
+}
+
+int64_t env_int(const char *name, int64_t dflt)
+{
+    const char *v = getenv(name);
+    if (!v || !*v) return dflt;
+    return strtoll(v, nullptr, 10);
+}
+
+// ---------------------------------------------------------------------------
+// Plan representation
+// ---------------------------------------------------------------------------
+enum BufId { B_IN = 0, B_OUT = 1, B_SCR = 2 };
+enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY };
+
+struct Launch {
+    LaunchType type;
+    const KernelEntry *k = nullptr;
+    TileArgs ta{};
+    UnpackArgs ua{};
+    HFillArgs ha{};
+    CopyArgs ca{};
+    int in_buf = B_IN, out_buf = B_OUT;
+    int64_t in_off = 0, out_off = 0;  // bytes
+    bool vin_ok = false, vout_ok = false;
+    bool first = false, last = false;  // ends of a complex chain (inverse swaps)
+    int64_t grid = 0;
+};
+
+}  // namespace
+}  // namespace gfft
+
+struct gfft_plan_s {
+    gfft_desc d{};
+    int prec = 0;
+    int device = 0;
+    std::vector<gfft::Launch> launches;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int dir_mask = 3;  // bit0 forward allowed, bit1 inverse allowed
+    size_t in_bytes = 0, out_bytes = 0;
+    bool out_has_gaps = false;
+    std::mutex host_mu;
+    void *hin = nullptr, *hout = nullptr;
+};
+
+namespace gfft {
+namespace {
+
+struct Builder {
+    gfft_plan_s *p;
+    int prec;
+    size_t csize;  // bytes of one complex element
+    bool dry;      // plan without a device: no table uploads
+    int64_t scratch_elems = 0;  // complex elements of scratch needed
+
+    void need_scratch(int64_t elems) { scratch_elems = std::max(scratch_elems, elems); }
+
+    // one tile pass
+    int tile(int L, int inp, int outp, int prefer_m, int in_buf, int64_t in_off_el, int out_buf,
+             int64_t out_off_el, int64_t n_inner, int64_t n_outer1, int64_t n_outer2,
+             int64_t in_outer2, int64_t in_outer, int64_t in_inner, int64_t in_elem,
+             int64_t out_outer2, int64_t out_outer, int64_t out_inner, int64_t out_elem,
+             int tw_mode, int64_t tw_n, bool first, bool last)
+    {
+        const KernelEntry *k = find_tile(prec, L, inp, outp, prefer_m);
+        if (!k) return fail(GFFT_ERR_UNSUPPORTED, "no tile kernel for L=%d pattern %d->%d", L, inp, outp);
+        Launch l;
+        l.type = LT_TILE;
+        l.k = k;
+        TileArgs &a = l.ta;
+        a.in_outer2 = in_outer2;
+        a.in_outer = in_outer;
+        a.in_inner = in_inner;
+        a.in_elem = in_elem;
+        a.out_outer2 = out_outer2;
+        a.out_outer = out_outer;
+        a.out_inner = out_inner;
+        a.out_elem = out_elem;
+        const int64_t tiles_inner = (n_inner + k->M - 1) / k->M;
+        const int64_t ntiles = tiles_inner * n_outer1 * n_outer2;
+        if (ntiles > INT32_MAX || n_inner > INT32_MAX)
+            return fail(GFFT_ERR_UNSUPPORTED, "too many tiles (%lld)", (long long)ntiles);
+        a.n_inner = (int32_t)n_inner;
+        a.tiles_inner = (int32_t)tiles_inner;
+        a.n_outer1 = (int32_t)n_outer1;
+        a.n_tiles = (int32_t)ntiles;
+        a.tw_mode = tw_mode;
+        if (tw_mode != TW_NONE) {
+            TwTable t;
+            int st = get_two_level(p->device, tw_n, &t, dry);
+            if (st) return st;
+            a.tw_lo = t.lo;
+            a.tw_hi = t.hi;
+            a.tw_mask = (uint32_t)t.mask;
+            a.tw_lo_bits = t.lo_bits;
+        }
+        if (inp == PAT_ROW ? in_elem != 1 : in_inner != 1)
+            return fail(GFFT_ERR_UNSUPPORTED, "internal: input pattern does not match strides");
+        if (outp == PAT_ROW ? out_elem != 1 : out_inner != 1)
+            return fail(GFFT_ERR_UNSUPPORTED, "internal: output pattern does not match strides");
+        int st = get_pass_twiddles(p->device, *k, &a.pass_tw, dry);
+        if (st) return st;
+        // 16-byte vectors need even element offsets everywhere in the tile
+        auto even = [](int64_t v) { return (v & 1) == 0; };
+        if (prec == 0) {
+            l.vin_ok = even(in_outer2) && even(in_outer) && even(in_off_el) &&
+                       (inp == PAT_ROW ? even(in_inner) : even(in_elem));
+            l.vout_ok = even(out_outer2) && even(out_outer) && even(out_off_el) &&
+                        (outp == PAT_ROW ? even(out_inner) : even(out_elem));
+        }
+        l.in_buf = in_buf;
+        l.out_buf = out_buf;
+        l.in_off = in_off_el * (int64_t)csize;
+        l.out_off = out_off_el * (int64_t)csize;
+        l.first = first;
+        l.last = last;
+        l.grid = ntiles;
+        p->launches.push_back(l);
+        return GFFT_OK;
+    }
+
+    // Batched 1D complex transforms of length n; `batch` transforms at
+    // distances in_dist / out_dist (complex elements).
+    int c2c_1d(int64_t n, int64_t batch, int in_buf, int64_t in_off, int64_t in_dist, int out_buf,
+               int64_t out_off, int64_t out_dist, bool first, bool last)
+    {
+        if (n == 1) {
+            Launch l;
+            l.type = LT_COPY;
+            l.ca.batch = batch;
+            l.ca.in_dist = in_dist;
+            l.ca.out_dist = out_dist;
+            l.in_buf = in_buf;
+            l.out_buf = out_buf;
+            l.in_off = in_off * (int64_t)csize;
+            l.out_off = out_off * (int64_t)csize;
+            l.first = first;
+            l.last = last;
+            p->launches.push_back(l);
+            return GFFT_OK;
+        }
+        const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
+        if (n <= maxrow)
+            return tile((int)n, PAT_ROW, PAT_ROW, 0, in_buf, in_off, out_buf, out_off, batch, 1, 1, 0, 0,
+                        in_dist, 1, 0, 0, out_dist, 1, TW_NONE, 0, first, last);
+        // four-step: n = N1 (column length) * N2 (row length)
+        const int lg = ilog2(n);
+        int l2 = lg / 2;                       // row pass length 2^l2 .. keep N1 <= N2
+        int l1 = lg - l2;
+        if (l1 > l2) std::swap(l1, l2);
+        const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
+        if (N1 > max_len(prec, PAT_COL, PAT_COL) || N2 > max_len(prec, PAT_ROW, PAT_COL))
+            return fail(GFFT_ERR_UNSUPPORTED, "length %lld exceeds the two-pass limit", (long long)n);
+        const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)64 << 20);
+        int64_t chunk = std::max<int64_t>(1, chunk_bytes / (n * (int64_t)csize));
+        chunk = std::min(chunk, batch);
+        need_scratch(chunk * n);
+        const int pm1 = (int)env_int("GFFT_COL_M", 0), pm2 = (int)env_int("GFFT_ROW_M", 0);
+        for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
+            const int64_t cb = std::min(chunk, batch - b0);
+            int st = tile((int)N1, PAT_COL, PAT_COL, pm1, in_buf, in_off + b0 * in_dist, B_SCR, 0, N2, cb, 1, 0,
+                          in_dist, 1, N2, 0, n, 1, N2, TW_INNER, n, first, false);
+            if (st) return st;
+            st = tile((int)N2, PAT_ROW, PAT_COL, pm2, B_SCR, 0, out_buf, out_off + b0 * out_dist, N1, cb, 1, 0,
+                      n, N2, 1, 0, out_dist, 1, N1, TW_NONE, 0, false, last);
+            if (st) return st;
+        }
+        return GFFT_OK;
+    }
+
+    // Column FFTs of length `rows` down `cols` columns, `batch` matrices.
+    // in/out pitch (row stride) and batch distance in complex elements.
+    int columns(int64_t rows, int64_t cols, int64_t batch, int in_buf, int64_t in_pitch, int64_t in_dist,
+                int out_buf, int64_t out_pitch, int64_t out_dist, bool first, bool last)
+    {
+        if (rows == 1) {
+            if (in_buf == out_buf && in_pitch == out_pitch && in_dist == out_dist) return GFFT_OK;
+            // identity copy of the rows: run as length-1 "rows" of a copy
+            for (int64_t c = 0; c < cols; c++) {
+                Launch l;
+                l.type = LT_COPY;
+                l.ca.batch = batch;
+                l.ca.in_dist = in_dist;
+                l.ca.out_dist = out_dist;
+                l.in_buf = in_buf;
+                l.out_buf = out_buf;
+                l.in_off = c * (int64_t)csize;
+                l.out_off = c * (int64_t)csize;
+                p->launches.push_back(l);
+            }
+            return GFFT_OK;
+        }
+        const int maxcol = max_len(prec, PAT_COL, PAT_COL);
+        if (rows <= maxcol)
+            return tile((int)rows, PAT_COL, PAT_COL, 0, in_buf, 0, out_buf, 0, cols, batch, 1, 0, in_dist, 1,
+                        in_pitch, 0, out_dist, 1, out_pitch, TW_NONE, 0, first, last);
+        // rows = A * B, sub-pass 1 over n2 (length B, rows n1 + A*n2) with
+        // twiddle W_rows^(n1*k2) into scratch, sub-pass 2 over n1 (length A)
+        // writing row k2 + B*k1.
+        const int lg = ilog2(rows);
+        const int lb = lg / 2, la = lg - lb;
+        const int64_t A = (int64_t)1 << la, B = (int64_t)1 << lb;
+        if (A > maxcol) return fail(GFFT_ERR_UNSUPPORTED, "column length %lld too long", (long long)rows);
+        const int64_t spitch = cols, sdist = rows * cols;
+        need_scratch(sdist * batch);
+        int st = tile((int)B, PAT_COL, PAT_COL, 0, in_buf, 0, B_SCR, 0, cols, A, batch, in_dist, in_pitch, 1,
+                      A * in_pitch, sdist, spitch, 1, A * spitch, TW_OUTER, rows, first, false);
+        if (st) return st;
+        return tile((int)A, PAT_COL, PAT_COL, 0, B_SCR, 0, out_buf, 0, cols, B, batch, sdist, A * spitch, 1,
+                    spitch, out_dist, out_pitch, 1, B * out_pitch, TW_NONE, 0, false, last);
+    }
+
+    int unpack(int64_t n, int64_t batch, int in_buf, int64_t in_dist, int out_buf, int64_t out_dist, bool half,
+               bool pack)
+    {
+        Launch l;
+        l.type = pack ? LT_PACK : LT_UNPACK;
+        l.ua.n = n;
+        l.ua.batch = batch;
+        l.ua.in_dist = in_dist;
+        l.ua.out_dist = out_dist;
+        l.ua.half = half ? 1 : 0;
+        if (n >= 2) {
+            int st = get_two_level(p->device, n, &l.ua.tw, dry);
+            if (st) return st;
+        }
+        l.in_buf = in_buf;
+        l.out_buf = out_buf;
+        p->launches.push_back(l);
+        return GFFT_OK;
+    }
+};
+
+int64_t dflt(int64_t v, int64_t d) { return v ? v : d; }
+
+int build_plan(gfft_plan_s *p, bool dry)
+{
+    const gfft_desc &d = p->d;
+    Builder b{p, p->prec, (size_t)(p->prec ? 16 : 8), dry};
+    const size_t cs = b.csize, rs = cs / 2;
+    const int64_t n = d.n, batch = d.batch;
+    int st = GFFT_OK;
+    switch (d.kind) {
+    case GFFT_C2C: {
+        const int64_t id = dflt(d.in_dist, n), od = dflt(d.out_dist, n);
+        p->in_bytes = ((batch - 1) * id + n) * cs;
+        p->out_bytes = ((batch - 1) * od + n) * cs;
+        p->out_has_gaps = od > n;
+        st = b.c2c_1d(n, batch, B_IN, 0, id, B_OUT, 0, od, true, true);
+        break;
+    }
+    case GFFT_R2C: {
+        const bool half = d.flags & GFFT_FLAG_HALF;
+        const int64_t nout = half ? n / 2 + 1 : n;
+        const int64_t id = dflt(d.in_dist, n), od = dflt(d.out_dist, nout);
+        p->in_bytes = ((batch - 1) * id + n) * rs;
+        p->out_bytes = ((batch - 1) * od + nout) * cs;
+        p->out_has_gaps = od > nout;
+        p->dir_mask = 1;
+        if (n == 1) {
+            Launch l;
+            l.type = LT_COPY;
+            l.ca = {nullptr, nullptr, batch, id, od, 1, 0};
+            p->launches.push_back(l);
+            break;
+        }
+        if (id % 2) return fail(GFFT_ERR_UNSUPPORTED, "R2C input distance must be even");
+        st = b.c2c_1d(n / 2, batch, B_IN, 0, id / 2, B_OUT, 0, od, true, true);
+        if (!st) st = b.unpack(n, batch, B_OUT, od, B_OUT, od, half, false);
+        break;
+    }
+    case GFFT_C2R: {
+        const int64_t nin = n / 2 + 1;
+        const int64_t id = dflt(d.in_dist, nin), od = dflt(d.out_dist, n);
+        p->in_bytes = ((batch - 1) * id + nin) * cs;
+        p->out_bytes = ((batch - 1) * od + n) * rs;
+        p->out_has_gaps = od > n;
+        p->dir_mask = 2;
+        if (n == 1) {
+            Launch l;
+            l.type = LT_COPY;
+            l.ca = {nullptr, nullptr, batch, id, od, 0, 1};
+            p->launches.push_back(l);
+            break;
+        }
+        if (od % 2) return fail(GFFT_ERR_UNSUPPORTED, "C2R output distance must be even");
+        const int64_t L = n / 2;
+        b.need_scratch(L * batch);
+        st = b.unpack(n, batch, B_IN, id, B_SCR, L, false, true);
+        if (!st) st = b.c2c_1d(L, batch, B_SCR, 0, L, B_OUT, 0, od / 2, true, true);
+        // scratch is shared with a four-step c2c: the pack target must not
+        // overlap the four-step ring, so offset it past the ring.
+        if (!st && L > max_len(p->prec, PAT_ROW, PAT_ROW)) {
+            const int64_t ring = b.scratch_elems;
+            b.scratch_elems = ring + L * batch;
+            for (auto &l : p->launches) {
+                if (l.type == LT_PACK) l.out_off = ring * (int64_t)cs;
+                if (l.type == LT_TILE && l.first) l.in_off += ring * (int64_t)cs;
+            }
+        }
+        break;
+    }
+    case GFFT_DIT: {
+        const bool half = d.flags & GFFT_FLAG_HALF;
+        const int64_t nin = std::max<int64_t>(1, n / 2), nout = half ? n / 2 + 1 : n;
+        const int64_t id = dflt(d.in_dist, nin), od = dflt(d.out_dist, nout);
+        p->in_bytes = ((batch - 1) * id + nin) * cs;
+        p->out_bytes = ((batch - 1) * od + nout) * cs;
+        p->out_has_gaps = od > nout;
+        p->dir_mask = 1;
+        st = b.unpack(n, batch, B_IN, id, B_OUT, od, half, false);
+        break;
+    }
+    case GFFT_COLS: {
+        const int64_t rows = n, cols = d.n2;
+        const int64_t ip = dflt(d.in_stride, cols), op = dflt(d.out_stride, cols);
+        if (ip < cols || op < cols) return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the column count");
+        const int64_t id = dflt(d.in_dist, rows * ip), od = dflt(d.out_dist, rows * op);
+        p->in_bytes = ((batch - 1) * id + (rows - 1) * ip + cols) * cs;
+        p->out_bytes = ((batch - 1) * od + (rows - 1) * op + cols) * cs;
+        p->out_has_gaps = op > cols || od > rows * op;
+        st = b.columns(rows, cols, batch, B_IN, ip, id, B_OUT, op, od, true, true);
+        break;
+    }
+    case GFFT_C2C_2D: {
+        const int64_t W = n, H = d.n2;
+        const int64_t ip = dflt(d.in_stride, W), op = dflt(d.out_stride, W);
+        if (ip < W || op < W) return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the width");
+        const int64_t id = dflt(d.in_dist, H * ip), od = dflt(d.out_dist, H * op);
+        p->in_bytes = ((batch - 1) * id + (H - 1) * ip + W) * cs;
+        p->out_bytes = ((batch - 1) * od + (H - 1) * op + W) * cs;
+        p->out_has_gaps = op > W || od > H * op;
+        if (W > max_len(p->prec, PAT_ROW, PAT_ROW))
+            return fail(GFFT_ERR_UNSUPPORTED, "2D width %lld exceeds the single-pass row limit", (long long)W);
+        if (W == 1) {
+            st = b.columns(H, 1, batch, B_IN, ip, id, B_OUT, op, od, true, true);
+            break;
+        }
+        // rows: H rows of batch matrices (two outer levels)
+        st = b.tile((int)W, PAT_ROW, PAT_ROW, 0, B_IN, 0, B_OUT, 0, H, batch, 1, 0, id, ip, 1, 0, od, op, 1,
+                    TW_NONE, 0, true, H == 1);
+        if (!st && H > 1) st = b.columns(H, W, batch, B_OUT, op, od, B_OUT, op, od, false, true);
+        break;
+    }
+    case GFFT_R2C_2D: {
+        const int64_t M = n, N = d.n2;
+        const int64_t ip = dflt(d.in_stride, M), op = dflt(d.out_stride, M);
+        if (ip < M || op < M) return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the width");
+        const int64_t id = dflt(d.in_dist, N * ip), od = dflt(d.out_dist, N * op);
+        p->in_bytes = ((batch - 1) * id + (N - 1) * ip + M) * rs;
+        p->out_bytes = ((batch - 1) * od + (N - 1) * op + M) * cs;
+        p->out_has_gaps = op > M || od > N * op;
+        p->dir_mask = 1;
+        if (M == 1) {
+            // a single real column: complex column FFT of (x, 0)
+            return fail(GFFT_ERR_UNSUPPORTED, "R2C_2D needs width >= 2");
+        }
+        if ((ip % 2) || (id % 2)) return fail(GFFT_ERR_UNSUPPORTED, "R2C_2D input pitch must be even");
+        const int64_t half = M / 2;
+        // row r2c (half spectrum, M/2+1 bins) on every row of every matrix
+        if (batch == 1) {
+            st = b.c2c_1d(half, N, B_IN, 0, ip / 2, B_OUT, 0, op, true, true);
+            if (!st) st = b.unpack(M, N, B_OUT, op, B_OUT, op, true, false);
+        } else {
+            for (int64_t m = 0; m < batch && !st; m++) {
+                const size_t before = p->launches.size();
+                st = b.c2c_1d(half, N, B_IN, m * (id / 2), ip / 2, B_OUT, m * od, op, true, true);
+                if (st) break;
+                st = b.unpack(M, N, B_OUT, op, B_OUT, op, true, false);
+                if (st) break;
+                p->launches.back().in_off = p->launches.back().out_off = m * od * (int64_t)cs;
+                (void)before;
+            }
+        }
+        const int64_t vcols = std::min<int64_t>(M / 2 + 1, M);
+        if (!st && N > 1) st = b.columns(N, vcols, batch, B_OUT, op, od, B_OUT, op, od, true, true);
+        if (!st && vcols < M) {
+            Launch l;
+            l.type = LT_HFILL;
+            l.ha = {nullptr, M, N, vcols, op, od, batch};
+            l.in_buf = l.out_buf = B_OUT;
+            p->launches.push_back(l);
+        }
+        break;
+    }
+    default:
+        return fail(GFFT_ERR_INVALID_ARG, "unknown transform kind %d", d.kind);
+    }
+    if (st) return st;
+    b.scratch_elems = std::max<int64_t>(b.scratch_elems, 0);
+    p->scratch_bytes = (size_t)b.scratch_elems * cs;
+    return GFFT_OK;
+}
+
+// Plan-time bounds proof: the largest byte offset each launch can touch,
+// from the same index formulas the kernels use, must lie inside its buffer.
+// A planner mistake becomes an error here instead of a device fault.
+int validate(const gfft_plan_s *p)
+{
+    const int64_t cs = p->prec ? 16 : 8, rs = cs / 2;
+    const int64_t ext[3] = {(int64_t)p->in_bytes, (int64_t)p->out_bytes, (int64_t)p->scratch_bytes};
+    auto inside = [&](int buf, int64_t off, int64_t last_byte, const char *what) {
+        if (off < 0 || last_byte < 0 || off + last_byte > ext[buf])
+            return fail(GFFT_ERR_UNSUPPORTED,
+                        "internal: %s reaches byte %lld of buffer %d (extent %lld)", what,
+                        (long long)(off + last_byte), buf, (long long)ext[buf]);
+        return GFFT_OK;
+    };
+    int idx = 0;
+    for (const Launch &l : p->launches) {
+        int st = GFFT_OK;
+        idx++;
+        switch (l.type) {
+        case LT_TILE: {
+            const TileArgs &a = l.ta;
+            const int L = l.k->L;
+            const int64_t n_outer2 = a.n_tiles / ((int64_t)a.tiles_inner * a.n_outer1);
+            if ((int64_t)a.tiles_inner * a.n_outer1 * n_outer2 != a.n_tiles)
+                return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d tile count", idx);
+            const int64_t li = (n_outer2 - 1) * a.in_outer2 + (int64_t)(a.n_outer1 - 1) * a.in_outer +
+                               (int64_t)(a.n_inner - 1) * a.in_inner + (int64_t)(L - 1) * a.in_elem;
+            const int64_t lo = (n_outer2 - 1) * a.out_outer2 + (int64_t)(a.n_outer1 - 1) * a.out_outer +
+                               (int64_t)(a.n_inner - 1) * a.out_inner + (int64_t)(L - 1) * a.out_elem;
+            if ((st = inside(l.in_buf, l.in_off, (li + 1) * cs, "tile load"))) return st;
+            if ((st = inside(l.out_buf, l.out_off, (lo + 1) * cs, "tile store"))) return st;
+            if (l.k->lds_bytes > 160 * 1024) return fail(GFFT_ERR_UNSUPPORTED, "internal: LDS");
+            break;
+        }
+        case LT_UNPACK: {
+            const UnpackArgs &a = l.ua;
+            const int64_t nin = std::max<int64_t>(1, a.n / 2), nout = a.half ? a.n / 2 + 1 : std::max<int64_t>(1, a.n);
+            if ((st = inside(l.in_buf, l.in_off, ((a.batch - 1) * a.in_dist + nin) * cs, "unpack load"))) return st;
+            if ((st = inside(l.out_buf, l.out_off, ((a.batch - 1) * a.out_dist + nout) * cs, "unpack store")))
+                return st;
+            break;
+        }
+        case LT_PACK: {
+            const UnpackArgs &a = l.ua;
+            if ((st = inside(l.in_buf, l.in_off, ((a.batch - 1) * a.in_dist + a.n / 2 + 1) * cs, "pack load")))
+                return st;
+            if ((st = inside(l.out_buf, l.out_off, ((a.batch - 1) * a.out_dist + a.n / 2) * cs, "pack store")))
+                return st;
+            break;
+        }
+        case LT_HFILL: {
+            const HFillArgs &a = l.ha;
+            if ((st = inside(l.out_buf, l.out_off,
+                             ((a.batch - 1) * a.dist + (a.N - 1) * a.stride + a.M) * cs, "hfill")))
+                return st;
+            break;
+        }
+        case LT_COPY: {
+            const CopyArgs &a = l.ca;
+            const int64_t isz = a.real_in ? rs : cs, osz = a.real_out ? rs : cs;
+            if ((st = inside(l.in_buf, l.in_off, ((a.batch - 1) * a.in_dist + 1) * isz, "copy load"))) return st;
+            if ((st = inside(l.out_buf, l.out_off, ((a.batch - 1) * a.out_dist + 1) * osz, "copy store")))
+                return st;
+            break;
+        }
+        }
+    }
+    return GFFT_OK;
+}
+
+int describe(const gfft_plan_s *p, char *buf, size_t len)
+{
+    std::string s;
+    char line[1024];
+    const char *bn[3] = {"in", "out", "scratch"};
+    snprintf(line, sizeof(line), "plan in_bytes=%zu out_bytes=%zu scratch_bytes=%zu\n", p->in_bytes,
+             p->out_bytes, p->scratch_bytes);
+    s += line;
+    for (const Launch &l : p->launches) {
+        switch (l.type) {
+        case LT_TILE: {
+            const TileArgs &a = l.ta;
+            snprintf(line, sizeof(line),
+                     "tile L=%d M=%d NT=%d layout=%d inp=%d outp=%d lds=%d rad=%d,%d,%d,%d tiles=%d "
+                     "n_inner=%d tiles_inner=%d n_outer1=%d in=%s+%lld out=%s+%lld "
+                     "in_strides=%lld,%lld,%lld,%lld out_strides=%lld,%lld,%lld,%lld tw=%d first=%d last=%d\n",
+                     l.k->L, l.k->M, l.k->NT, l.k->layout, l.k->inp, l.k->outp, l.k->lds_bytes, l.k->rad[0],
+                     l.k->nrad > 1 ? l.k->rad[1] : 0, l.k->nrad > 2 ? l.k->rad[2] : 0,
+                     l.k->nrad > 3 ? l.k->rad[3] : 0, a.n_tiles, a.n_inner, a.tiles_inner, a.n_outer1,
+                     bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off,
+                     (long long)a.in_outer2, (long long)a.in_outer, (long long)a.in_inner, (long long)a.in_elem,
+                     (long long)a.out_outer2, (long long)a.out_outer, (long long)a.out_inner,
+                     (long long)a.out_elem, a.tw_mode, (int)l.first, (int)l.last);
+            break;
+        }
+        case LT_UNPACK:
+        case LT_PACK:
+            snprintf(line, sizeof(line), "%s n=%lld batch=%lld in=%s+%lld out=%s+%lld in_dist=%lld out_dist=%lld "
+                     "half=%d\n", l.type == LT_PACK ? "pack" : "unpack", (long long)l.ua.n, (long long)l.ua.batch,
+                     bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off,
+                     (long long)l.ua.in_dist, (long long)l.ua.out_dist, l.ua.half);
+            break;
+        case LT_HFILL:
+            snprintf(line, sizeof(line), "hfill M=%lld N=%lld vcols=%lld stride=%lld dist=%lld batch=%lld\n",
+                     (long long)l.ha.M, (long long)l.ha.N, (long long)l.ha.vcols, (long long)l.ha.stride,
+                     (long long)l.ha.dist, (long long)l.ha.batch);
+            break;
+        case LT_COPY:
+            snprintf(line, sizeof(line), "copy batch=%lld in=%s+%lld out=%s+%lld\n", (long long)l.ca.batch,
+                     bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off);
+            break;
+        }
+        s += line;
+    }
+    if (s.size() + 1 > len) return fail(GFFT_ERR_INVALID_ARG, "describe buffer too small (%zu bytes needed)", s.size() + 1);
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return GFFT_OK;
+}
+
+// Executes the plan's launches in order on stream s.  With `events`, records
+// events[2i] / events[2i+1] on s around launch i (per-kernel timing).
+int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
+        hipEvent_t *events = nullptr)
+{
+    const bool inv = direction == GFFT_INVERSE;
+    char *bufs[3] = {(char *)in, (char *)out, (char *)p->scratch};
+    int li = 0;
+    for (const Launch &L0 : p->launches) {
+        Launch l = L0;
+        if (events) HIP_TRY(hipEventRecord(events[2 * li], s));
+        char *ip = bufs[l.in_buf] + l.in_off;
+        char *op = bufs[l.out_buf] + l.out_off;
+        switch (l.type) {
+        case LT_TILE: {
+            TileArgs a = l.ta;
+            a.in = ip;
+            a.out = op;
+            a.swap_in = inv && l.first;
+            a.swap_out = inv && l.last;
+            a.vec_in = l.vin_ok && (((uintptr_t)ip & 15) == 0);
+            a.vec_out = l.vout_ok && (((uintptr_t)op & 15) == 0);
+            void *args[] = {&a};
+            HIP_TRY(hipLaunchKernel((const void *)l.k->fn, dim3((unsigned)l.grid), dim3(l.k->NT), args, 0, s));
+            break;
+        }
+        case LT_UNPACK:
+        case LT_PACK: {
+            UnpackArgs a = l.ua;
+            a.in = ip;
+            a.out = op;
+            HIP_TRY(l.type == LT_UNPACK ? launch_unpack(p->prec, a, s) : launch_pack(p->prec, a, s));
+            break;
+        }
+        case LT_HFILL: {
+            HFillArgs a = l.ha;
+            a.out = op;
+            HIP_TRY(launch_hfill(p->prec, a, s));
+            break;
+        }
+        case LT_COPY: {
+            CopyArgs a = l.ca;
+            a.in = ip;
+            a.out = op;
+            HIP_TRY(launch_copy(p->prec, a, s));
+            break;
+        }
+        }
+        if (events) HIP_TRY(hipEventRecord(events[2 * li + 1], s));
+        li++;
+    }
+    return GFFT_OK;
+}
+
+bool device_ok(int *dev)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return false;
+    return hipGetDevice(dev) == hipSuccess;
+}
+
+}  // namespace
+}  // namespace gfft
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace gfft;
+
+extern "C" {
+
+int gfft_version(void) { return GFFT_VERSION; }
+
+const char *gfft_error_string(int status)
+{
+    switch (status) {
+    case GFFT_OK: return "success";
+    case GFFT_ERR_INVALID_ARG: return "invalid argument";
+    case GFFT_ERR_UNSUPPORTED: return "unsupported size or layout";
+    case GFFT_ERR_HIP: return "HIP runtime error";
+    case GFFT_ERR_ALLOC: return "allocation failed";
+    case GFFT_ERR_NO_DEVICE: return "no usable GPU device";
+    default: return "unknown status";
+    }
+}
+
+const char *gfft_last_error(void) { return g_last_error.c_str(); }
+
+int gfft_is_device_pointer(const void *ptr)
+{
+    if (!ptr) return 0;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+static int check_desc(const gfft_desc &d)
+{
+    if (d.precision != GFFT_F32 && d.precision != GFFT_F64)
+        return fail(GFFT_ERR_INVALID_ARG, "precision must be GFFT_F32 or GFFT_F64");
+    if (d.kind < GFFT_C2C || d.kind > GFFT_C2R) return fail(GFFT_ERR_INVALID_ARG, "unknown kind %d", d.kind);
+    if (d.batch < 1) return fail(GFFT_ERR_INVALID_ARG, "batch must be >= 1");
+    if (!is_pow2(d.n) || d.n > ((int64_t)1 << 23))
+        return fail(GFFT_ERR_UNSUPPORTED, "size %lld is not a power of two in 1..2^23", (long long)d.n);
+    const bool two = d.kind == GFFT_C2C_2D || d.kind == GFFT_R2C_2D || d.kind == GFFT_COLS;
+    if (two && d.kind != GFFT_COLS && (!is_pow2(d.n2) || d.n2 > ((int64_t)1 << 23)))
+        return fail(GFFT_ERR_UNSUPPORTED, "height %lld is not a power of two", (long long)d.n2);
+    if (d.kind == GFFT_COLS && d.n2 < 1) return fail(GFFT_ERR_INVALID_ARG, "column count must be >= 1");
+    if (d.in_stride < 0 || d.out_stride < 0 || d.in_dist < 0 || d.out_dist < 0)
+        return fail(GFFT_ERR_INVALID_ARG, "negative stride or distance");
+    if (d.reserved != 0) return fail(GFFT_ERR_INVALID_ARG, "reserved field must be 0");
+    const int64_t width = d.kind == GFFT_COLS ? d.n2 : d.n;
+    if (two && ((d.in_stride && d.in_stride < width) || (d.out_stride && d.out_stride < width)))
+        return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the row length");
+    return GFFT_OK;
+}
+
+int gfft_plan_explain(const gfft_desc *desc, char *buf, size_t len)
+{
+    if (!desc || !buf || !len) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    int st = check_desc(*desc);
+    if (st) return st;
+    std::unique_ptr<gfft_plan_s> p(new gfft_plan_s());
+    p->d = *desc;
+    p->prec = desc->precision;
+    if ((st = build_plan(p.get(), true))) return st;
+    if ((st = validate(p.get()))) return st;
+    return describe(p.get(), buf, len);
+}
+
+int gfft_plan_create(gfft_plan *plan, const gfft_desc *desc)
+{
+    if (!plan || !desc) return fail(GFFT_ERR_INVALID_ARG, "null plan or descriptor");
+    *plan = nullptr;
+    const gfft_desc &d = *desc;
+    int st = check_desc(d);
+    if (st) return st;
+    int dev = 0;
+    if (!device_ok(&dev)) return fail(GFFT_ERR_NO_DEVICE, "no HIP device available");
+    std::unique_ptr<gfft_plan_s> p(new gfft_plan_s());
+    p->d = d;
+    p->prec = d.precision;
+    p->device = dev;
+    if ((st = build_plan(p.get(), false))) return st;
+    if ((st = validate(p.get()))) return st;
+    if (p->scratch_bytes) {
+        hipError_t e = hipMalloc(&p->scratch, p->scratch_bytes);
+        if (e != hipSuccess)
+            return fail(GFFT_ERR_ALLOC, "scratch of %zu bytes: %s", p->scratch_bytes, hipGetErrorString(e));
+    }
+    *plan = p.release();
+    return GFFT_OK;
+}
+
+int gfft_plan_destroy(gfft_plan plan)
+{
+    if (!plan) return GFFT_OK;
+    if (plan->scratch) (void)hipFree(plan->scratch);
+    if (plan->hin) (void)hipFree(plan->hin);
+    if (plan->hout) (void)hipFree(plan->hout);
+    delete plan;
+    return GFFT_OK;
+}
+
+int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void *stream)
+{
+    if (!plan) return fail(GFFT_ERR_INVALID_ARG, "null plan");
+    if (!in || !out) return fail(GFFT_ERR_INVALID_ARG, "null buffer");
+    if (direction != GFFT_FORWARD && direction != GFFT_INVERSE)
+        return fail(GFFT_ERR_INVALID_ARG, "direction must be GFFT_FORWARD or GFFT_INVERSE");
+    if (!(plan->dir_mask & (1 << direction)))
+        return fail(GFFT_ERR_INVALID_ARG, "this plan kind supports only the %s direction",
+                    plan->dir_mask == 1 ? "forward" : "inverse");
+    return run(plan, in, out, direction, (hipStream_t)stream);
+}
+
+int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)
+{
+    if (!plan) return fail(GFFT_ERR_INVALID_ARG, "null plan");
+    if (!in || !out) return fail(GFFT_ERR_INVALID_ARG, "null buffer");
+    std::lock_guard<std::mutex> lk(plan->host_mu);
+    if (!plan->hin) {
+        if (hipMalloc(&plan->hin, plan->in_bytes) != hipSuccess)
+            return fail(GFFT_ERR_ALLOC, "device staging of %zu bytes", plan->in_bytes);
+        if (hipMalloc(&plan->hout, plan->out_bytes) != hipSuccess)
+            return fail(GFFT_ERR_ALLOC, "device staging of %zu bytes", plan->out_bytes);
+    }
+    HIP_TRY(hipMemcpy(plan->hin, in, plan->in_bytes, hipMemcpyHostToDevice));
+    // keep the bytes the transform does not write (pitch gaps) intact
+    if (plan->out_has_gaps) HIP_TRY(hipMemcpy(plan->hout, out, plan->out_bytes, hipMemcpyHostToDevice));
+    int st = gfft_execute(plan, plan->hin, plan->hout, direction, nullptr);
+    if (st) return st;
+    HIP_TRY(hipMemcpy(out, plan->hout, plan->out_bytes, hipMemcpyDeviceToHost));
+    return GFFT_OK;
+}
+
+int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int direction, void *stream, float *ms,
+                      int max_launches, int *count)
+{
+    if (!plan || !in || !out || !count) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    const int n = (int)plan->launches.size();
+    *count = n;
+    std::vector<hipEvent_t> ev(2 * n);
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    int st = run(plan, in, out, direction, (hipStream_t)stream, ev.data());
+    if (!st) {
+        hipError_t e = hipEventSynchronize(ev[2 * n - 1]);
+        if (e != hipSuccess) st = fail(GFFT_ERR_HIP, "hipEventSynchronize: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; !st && ms && i < n && i < max_launches; i++) {
+        hipError_t e = hipEventElapsedTime(&ms[i], ev[2 * i], ev[2 * i + 1]);
+        if (e != hipSuccess) st = fail(GFFT_ERR_HIP, "hipEventElapsedTime: %s", hipGetErrorString(e));
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return st;
+}
+
+int gfft_plan_workspace(gfft_plan plan, size_t *bytes)
+{
+    if (!plan || !bytes) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    *bytes = plan->scratch_bytes;
+    return GFFT_OK;
+}
+
+int gfft_plan_describe(gfft_plan plan, char *buf, size_t len)
+{
+    if (!plan || !buf || !len) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    return describe(plan, buf, len);
+}
+
+int gfft_separate_2x_real(int precision, void *out1, void *out2, const void *in, int64_t n, int64_t batch,
+                          void *stream)
+{
+    if (!out1 || !out2 || !in || n < 1 || batch < 1) return fail(GFFT_ERR_INVALID_ARG, "bad argument");
+    SeparateArgs a{in, out1, out2, n, batch, n};
+    HIP_TRY(launch_separate(precision, a, (hipStream_t)stream));
+    return GFFT_OK;
+}
+
+int gfft_interleave_real(int precision, void *out, const void *in1, const void *in2, int64_t n, int64_t batch,
+                         void *stream)
+{
+    if (!out || !in1 || !in2 || n < 1 || batch < 1) return fail(GFFT_ERR_INVALID_ARG, "bad argument");
+    InterleaveArgs a{in1, in2, out, n, batch};
+    HIP_TRY(launch_interleave(precision, a, (hipStream_t)stream));
+    return GFFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+/*
+ * genfft_amd.h -- C ABI of the MI355X (gfx950) FFT engine.
+ *
+ * This is the drop-in boundary under the genfft-compatible C++ surface in
+ * <genfft_amd/fft.h> (and any other binding: ctypes, cgo, JNI ...).  Plain C,
+ * plain pointers and sizes; every entry point returns an int status
+ * (GFFT_OK = 0) and records a message readable with gfft_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the genFFT tree):
+ *   genfft::FFT<T>(n)                 include/genFFT/fft.h:60-64
+ *        -> gfft_plan_create(kind = GFFT_C2C)
+ *   FFT<T>::transform<inv>(out, in)   include/genFFT/fft.h:80-85
+ *        -> gfft_execute(plan, in, out, inv, stream)
+ *   FFT<T>::transform_real(out, in)   include/genFFT/fft.h:90-94
+ *        -> GFFT_R2C plan without GFFT_FLAG_HALF
+ *   FFT<T>::transform_interleave      include/genFFT/fft.h:100-105
+ *        -> gfft_interleave_real() + GFFT_C2C execute
+ *   RealFFT<T>::forward(out,in,half)  include/genFFT/FFTReal.h:204-213
+ *        -> GFFT_R2C plan (GFFT_FLAG_HALF selects n/2+1 bins)
+ *   DIT<T>::apply(out, in, half)      include/genFFT/fft.h:180-188,
+ *                                     generic/fft_dit_impl_generic.inl:27-61
+ *        -> GFFT_DIT plan
+ *   separate_2x_real_FFT              include/genFFT/FFTReal.h:36-66
+ *        -> gfft_separate_2x_real()
+ *   FFTVert<T>::transform<inv>        include/genFFT/fft.h:146-163
+ *        -> GFFT_COLS plan
+ *   FFT2D<T>::transform<inv>          include/genFFT/fft.h:214-218
+ *        -> GFFT_C2C_2D plan
+ *   RealFFT2D<T>::forward             include/genFFT/FFTReal.h:83-104
+ *        -> GFFT_R2C_2D plan
+ *   backend::GetImpl / GetVertImpl / GetDITImpl (the plan factory seam,
+ *   include/genFFT/FFTBackend.h:40-70) -> the plan cache of the C++ layer.
+ *
+ * Conventions (same as the reference): forward kernel exp(-2*pi*i*j*k/N),
+ * inverse unscaled (inverse(forward(x)) = N*x), natural order in and out,
+ * interleaved complex (re, im), power-of-two sizes 2^0 .. 2^23.
+ */
+#ifndef GENFFT_AMD_H
+#define GENFFT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef GFFT_API
+#define GFFT_API __attribute__((visibility("default")))
+#endif
+
+#define GFFT_VERSION 1
+
+/* status codes */
+#define GFFT_OK 0
+#define GFFT_ERR_INVALID_ARG 1     /* null pointer, bad kind/precision/stride */
+#define GFFT_ERR_UNSUPPORTED 2     /* size or layout outside what the engine supports */
+#define GFFT_ERR_HIP 3             /* a HIP runtime call failed */
+#define GFFT_ERR_ALLOC 4           /* device or host allocation failed */
+#define GFFT_ERR_NO_DEVICE 5       /* no usable gfx950 device */
+
+/* transform kinds */
+#define GFFT_C2C 0      /* batched 1D complex -> complex                          */
+#define GFFT_R2C 1      /* batched 1D real -> complex (RealFFT::forward)          */
+#define GFFT_C2C_2D 2   /* 2D complex (FFT2D), n = width (cols), n2 = height      */
+#define GFFT_R2C_2D 3   /* 2D real-input forward (RealFFT2D::forward)             */
+#define GFFT_COLS 4     /* column FFTs (FFTVert): n = column length, n2 = columns */
+#define GFFT_DIT 5      /* real-FFT unpack only (DIT::apply), n = real length     */
+#define GFFT_C2R 6      /* batched half-spectrum -> real, unscaled inverse         */
+
+/* precisions */
+#define GFFT_F32 0      /* complex64  (interleaved float)  */
+#define GFFT_F64 1      /* complex128 (interleaved double) */
+
+/* flags */
+#define GFFT_FLAG_HALF 1 /* R2C / DIT: write n/2+1 bins (else all n, Hermitian-filled) */
+
+/* directions */
+#define GFFT_FORWARD 0
+#define GFFT_INVERSE 1
+
+typedef struct gfft_plan_s *gfft_plan;
+
+typedef struct gfft_desc {
+    int32_t kind;       /* GFFT_C2C ... GFFT_C2R                                   */
+    int32_t precision;  /* GFFT_F32 / GFFT_F64                                      */
+    int64_t n;          /* 1D length; 2D width (columns); COLS column length (rows) */
+    int64_t n2;         /* 2D height (rows); COLS number of columns; else ignored   */
+    int64_t batch;      /* independent transforms (>= 1)                            */
+    int64_t in_stride;  /* 2D / COLS row pitch of the input: complex elements, but   */
+                        /* scalars for R2C_2D (as RealFFT2D::forward); 0 = dense     */
+    int64_t out_stride; /* 2D / COLS row pitch of the output, complex elements; 0 = dense */
+    int64_t in_dist;    /* distance between batch members, elements of the input   */
+                        /* type (complex, or real for R2C); 0 = dense               */
+    int64_t out_dist;   /* same for the output; 0 = dense                           */
+    int32_t flags;      /* GFFT_FLAG_*                                              */
+    int32_t reserved;   /* must be 0                                                */
+} gfft_desc;
+
+/* Creates a plan: validates the descriptor, selects kernels, uploads the
+ * FP64-generated twiddle tables and allocates the plan's device workspace. */
+GFFT_API int gfft_plan_create(gfft_plan *plan, const gfft_desc *desc);
+
+/* Releases a plan and its workspace. NULL is accepted. */
+GFFT_API int gfft_plan_destroy(gfft_plan plan);
+
+/* Executes on device-resident buffers, asynchronously on `stream`
+ * (a hipStream_t; NULL = the null stream).  in == out is allowed for every
+ * kind except GFFT_COLS / GFFT_C2C_2D / GFFT_R2C_2D with different pitches.
+ * direction: GFFT_FORWARD or GFFT_INVERSE (R2C / R2C_2D / DIT are forward
+ * only, C2R inverse only).  Calls that share a plan may not overlap in time
+ * (they share the plan's workspace). */
+GFFT_API int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void *stream);
+
+/* Same as gfft_execute for host buffers: stages through plan-owned device
+ * memory and returns after the result is back in `out` (synchronous). */
+GFFT_API int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction);
+
+/* Runs the plan once like gfft_execute, recording a HIP event pair on
+ * `stream` around each kernel launch, waits for it, and writes the duration
+ * of launch i (milliseconds) to ms[i] for i < max_launches.  *count receives
+ * the number of launches.  For benchmarks and profiles; no reference
+ * counterpart (genFFT has no timing hooks). */
+GFFT_API int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int direction, void *stream,
+                               float *ms, int max_launches, int *count);
+
+/* Device workspace the plan holds, in bytes. */
+GFFT_API int gfft_plan_workspace(gfft_plan plan, size_t *bytes);
+
+/* Human-readable list of the plan's kernel launches (for logs and tests). */
+GFFT_API int gfft_plan_describe(gfft_plan plan, char *buf, size_t len);
+
+/* Plans `desc` without touching a device (no allocation, no upload), proves
+ * every launch's index range lies inside its buffer, and writes the same
+ * description as gfft_plan_describe.  Usable on machines without a GPU. */
+GFFT_API int gfft_plan_explain(const gfft_desc *desc, char *buf, size_t len);
+
+/* separate_2x_real_FFT on device data: `in` holds the n-point FFT of
+ * x + i*y; out1 / out2 receive the n-point spectra of x and y.  in may alias
+ * out1.  batch transforms at distance n (complex elements). */
+GFFT_API int gfft_separate_2x_real(int precision, void *out1, void *out2, const void *in, int64_t n,
+                          int64_t batch, void *stream);
+
+/* transform_interleave's packing step: out[k] = in1[k] + i*in2[k], n reals
+ * each, batch rows at distance n. */
+GFFT_API int gfft_interleave_real(int precision, void *out, const void *in1, const void *in2, int64_t n,
+                         int64_t batch, void *stream);
+
+/* 1 if p is device (or managed) memory visible to the current device. */
+GFFT_API int gfft_is_device_pointer(const void *p);
+
+/* Static text for a status code; the last error message of this thread. */
+GFFT_API const char *gfft_error_string(int status);
+GFFT_API const char *gfft_last_error(void);
+
+GFFT_API int gfft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GENFFT_AMD_H */

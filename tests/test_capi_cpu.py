@@ -1,0 +1,159 @@
+"""CPU-side checks of the C ABI (no GPU needed).
+
+* the library loads and exports every function include/genfft_amd.h declares;
+* descriptor validation returns the documented status codes;
+* the planner (dry run, gfft_plan_explain) proves every launch in bounds for
+  the sizes the reference tests use and the benchmark shapes, and picks the
+  single-pass / four-step / sub-pass structure it documents.
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+from genfft_amd import _capi as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "genfft_amd.h")).read()
+    return sorted(set(re.findall(r"GFFT_API\s+(?:const\s+)?\w+\s*\*?\s*(gfft_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.lib()
+    declared = header_functions()
+    assert len(declared) >= 12
+    assert sorted(C.EXPORTS) == declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.gfft_version() == 1
+
+
+def test_only_the_c_abi_is_exported():
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", C.LIB_PATH], capture_output=True, text=True).stdout
+    funcs = [ln.split()[-1] for ln in out.splitlines() if " T " in ln]
+    assert sorted(f for f in funcs if f.startswith("gfft_")) == header_functions()
+    assert not [f for f in funcs if not f.startswith("gfft_") and not f.startswith("_")]
+
+
+def desc(kind=C.GFFT_C2C, prec=C.GFFT_F32, n=1024, n2=0, batch=1, ins=0, outs=0, ind=0, outd=0, flags=0):
+    return C.GfftDesc(kind, prec, n, n2, batch, ins, outs, ind, outd, flags, 0)
+
+
+def explain(d):
+    buf = ctypes.create_string_buffer(1 << 20)
+    st = C.lib().gfft_plan_explain(ctypes.byref(d), buf, len(buf))
+    return st, buf.value.decode()
+
+
+def test_status_codes_and_messages():
+    lib = C.lib()
+    for code in range(6):
+        assert lib.gfft_error_string(code)
+    h = ctypes.c_void_p()
+    assert lib.gfft_plan_create(ctypes.byref(h), None) == C.GFFT_ERR_INVALID_ARG
+    bad = [
+        (desc(n=1000), C.GFFT_ERR_UNSUPPORTED),               # not a power of two
+        (desc(n=1 << 24), C.GFFT_ERR_UNSUPPORTED),            # beyond 2^23
+        (desc(prec=7), C.GFFT_ERR_INVALID_ARG),
+        (desc(kind=42), C.GFFT_ERR_INVALID_ARG),
+        (desc(batch=0), C.GFFT_ERR_INVALID_ARG),
+        (desc(ind=-1), C.GFFT_ERR_INVALID_ARG),
+        (desc(kind=C.GFFT_C2C_2D, n=64, n2=48), C.GFFT_ERR_UNSUPPORTED),
+        (desc(kind=C.GFFT_COLS, n=64, n2=0), C.GFFT_ERR_INVALID_ARG),
+        (desc(kind=C.GFFT_C2C_2D, n=64, n2=64, ins=32), C.GFFT_ERR_INVALID_ARG),  # pitch < width
+    ]
+    for d, code in bad:
+        assert lib.gfft_plan_create(ctypes.byref(h), ctypes.byref(d)) == code
+        assert lib.gfft_last_error()
+        assert explain(d)[0] == code
+
+
+def test_plan_create_without_gpu_reports_no_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    d = desc()
+    assert C.lib().gfft_plan_create(ctypes.byref(h), ctypes.byref(d)) == C.GFFT_ERR_NO_DEVICE
+
+
+def tiles(text):
+    return [ln for ln in text.splitlines() if ln.startswith("tile")]
+
+
+def field(line, name):
+    return re.search(rf"\b{name}=([^ ]+)", line).group(1)
+
+
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
+@pytest.mark.parametrize("k", range(0, 24))
+def test_c2c_plans_in_bounds(prec, k):
+    n = 1 << k
+    batch = max(1, (1 << 27) // (n * (8 << prec)))  # the ~1 GiB benchmark batches
+    for b in sorted({1, 3, batch}):
+        st, text = explain(desc(prec=prec, n=n, batch=b))
+        assert st == C.GFFT_OK, text or C.lib().gfft_last_error()
+        t = tiles(text)
+        maxrow = 16384 if prec == C.GFFT_F32 else 8192
+        if n == 1:
+            assert "copy" in text and not t
+        elif n <= maxrow:
+            assert len(t) == 1 and field(t[0], "L") == str(n)
+        else:
+            assert len(t) % 2 == 0 and len(t) >= 2       # column pass + transposing row pass per chunk
+            L1, L2 = int(field(t[0], "L")), int(field(t[1], "L"))
+            assert L1 * L2 == n and field(t[0], "tw") == "1"
+            assert field(t[0], "inp") == "1" and field(t[1], "outp") == "1"
+
+
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
+def test_real_and_dit_plans_in_bounds(prec):
+    for k in range(0, 24):
+        n = 1 << k
+        for half in (0, 1):
+            for kind in (C.GFFT_R2C, C.GFFT_DIT, C.GFFT_C2R):
+                for b in (1, 5):
+                    st, text = explain(desc(kind=kind, prec=prec, n=n, batch=b, flags=half))
+                    assert st == C.GFFT_OK, (kind, n, half, C.lib().gfft_last_error())
+
+
+VERT_TABLE = [(1, 1), (1, 40), (2, 1), (2, 7), (2, 999), (4, 1), (4, 7), (4, 999), (8, 1), (8, 3), (8, 7),
+              (8, 33), (8, 47), (8, 1023), (16, 1), (16, 47), (16, 999), (256, 1), (256, 1023), (1024, 1),
+              (1024, 767), (65536, 1), (65536, 31)]  # test/test_dispatch.cpp:96-120
+
+
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
+@pytest.mark.parametrize("n,cols", VERT_TABLE)
+def test_cols_plans_in_bounds(prec, n, cols):
+    for ins, outs in ((0, 0), (cols + 3, cols + 1)):
+        st, text = explain(desc(kind=C.GFFT_COLS, prec=prec, n=n, n2=cols, ins=ins, outs=outs))
+        assert st == C.GFFT_OK, C.lib().gfft_last_error()
+
+
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
+@pytest.mark.parametrize("w,h", [(1, 1), (2, 1), (1, 8), (32, 16), (8, 64), (4096, 4096), (8192, 8192),
+                                 (16, 4096), (4096, 2)])
+def test_2d_plans_in_bounds(prec, w, h):
+    for kind in (C.GFFT_C2C_2D, C.GFFT_R2C_2D):
+        if kind == C.GFFT_R2C_2D and w < 2:
+            continue
+        for ins, outs, b in ((0, 0, 1), (w + 2, w + 6, 1), (0, 0, 3)):
+            st, text = explain(desc(kind=kind, prec=prec, n=w, n2=h, ins=ins, outs=outs, batch=b))
+            if prec == C.GFFT_F64 and w > 8192:
+                assert st == C.GFFT_ERR_UNSUPPORTED
+                continue
+            assert st == C.GFFT_OK, (kind, w, h, C.lib().gfft_last_error())
+
+
+def test_bench_shape_structure():
+    """complex64 N=2^20 x 128: 1024-point column pass (with twiddle) + 1024-point transposing row pass."""
+    st, text = explain(desc(n=1 << 20, batch=128))
+    assert st == 0
+    t = tiles(text)
+    assert field(t[0], "L") == "1024" and field(t[1], "L") == "1024"
+    assert "scratch_bytes=" in text
