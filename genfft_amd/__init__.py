@@ -106,7 +106,7 @@ class Plan:
         return v.value
 
     def describe(self) -> str:
-        buf = ctypes.create_string_buffer(8192)
+        buf = ctypes.create_string_buffer(1 << 20)
         C.check(self._lib.gfft_plan_describe(self._h, buf, len(buf)))
         return buf.value.decode()
 

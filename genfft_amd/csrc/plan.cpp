@@ -396,7 +396,11 @@ struct Builder {
         const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
         if (N1 > max_len(prec, PAT_COL, PAT_COL) || N2 > max_len(prec, PAT_ROW, PAT_COL))
             return fail(GFFT_ERR_UNSUPPORTED, "length %lld exceeds the two-pass limit", (long long)n);
-        const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)64 << 20);
+        // Measured on MI355X (c64 2^20 x 128): one launch per pass over the
+        // whole batch beats chunking through a small scratch ring (launch
+        // tails outweigh any Infinity-Cache reuse), so chunks only bound the
+        // scratch size.
+        const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
         int64_t chunk = std::max<int64_t>(1, chunk_bytes / (n * (int64_t)csize));
         chunk = std::min(chunk, batch);
         need_scratch(chunk * n);

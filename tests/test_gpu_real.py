@@ -1,0 +1,163 @@
+"""GPU parity of the real-input paths:
+
+  RealFFT<T>::forward(out, in, half)   include/genFFT/FFTReal.h:204-213
+  DIT<T>::apply(out, in, half)         include/genFFT/fft.h:180-188
+  FFT<T>::transform_real / _interleave include/genFFT/fft.h:90-105
+  separate_2x_real_FFT                 include/genFFT/FFTReal.h:36-66
+  c2r (half spectrum -> reals)          no reference counterpart (FFTReal.h:127-130
+                                       is a TODO); pinned by the round trip and
+                                       the oracle's inverse DFT
+
+Sizes follow test/test_real_fft.cpp:58-61 (2 .. 2^22, half and full) and
+test/test_dit.cpp:57-61.  The half-spectrum case keeps the reference's
+sentinel check: nothing past bin n/2 may be written (test/fft_test_impl.h:88-105).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1 << k for k in range(1, 23)]
+
+
+def _tol(prec, n):
+    return O.tol_c64(n) if prec == "f32" else O.tol_c128(n)
+
+
+def _dts(prec):
+    return (np.float32, np.complex64, torch.float32, torch.complex64) if prec == "f32" else \
+        (np.float64, np.complex128, torch.float64, torch.complex128)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("half", [False, True])
+def test_real_forward_vs_oracle(gpu, prec, n, half):
+    import genfft_amd as G
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    b = max(2, min(64, (1 << 16) // n))
+    rng = np.random.default_rng(n + half)
+    x = rng.uniform(-1, 1, (b, n)).astype(rdt)
+    nout = n // 2 + 1 if half else n
+    # each row of `out` holds n complex slots, like the reference test's buffer
+    out = torch.full((b, n), 43 + 21j, dtype=tcdt, device=gpu)
+    from genfft_amd import _capi as C
+    p = G.Plan(C.GFFT_R2C, n, precision=0 if prec == "f32" else 1, batch=b, half=half, out_dist=n)
+    p.execute(torch.from_numpy(x).to(gpu), out)
+    y = out.cpu().numpy()
+    rows = [0, b - 1]
+    ref = O.fft_ld(x[rows].astype(np.complex128))[:, :nout]
+    assert O.rel_l2_rows(y[rows, :nout], ref).max() <= _tol(prec, n)
+    assert np.all(y[:, nout:] == 43 + 21j), "write past the half spectrum"
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 1024])
+@pytest.mark.parametrize("half", [False, True])
+def test_real_forward_matches_genfft(gpu, golden, prec, n, half):
+    import genfft_amd as G
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    x = golden[f"real_{prec}_n{n}_in"]
+    nout = n // 2 + 1 if half else n
+    y = G.rfft(torch.from_numpy(x).to(gpu)[None], half=half).cpu().numpy()[0]
+    g = golden[f"real_{prec}_n{n}_{'half' if half else 'full'}_native"][:nout]
+    eps = 1e-8 + n * 1e-12 if prec == "f64" else 1e-5 + n * 1e-8
+    assert np.max(np.abs(y - g)) <= eps
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 64, 1024, 1 << 14, 1 << 20])
+@pytest.mark.parametrize("half", [False, True])
+def test_dit_apply(gpu, prec, n, half):
+    """DIT(n).apply on the n/2-point FFT of the packed signal equals the
+    full FFT of the real signal (test/fft_test_impl.h:60-82), in place too."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    rng = np.random.default_rng(5 + n)
+    x = rng.uniform(-1, 1, max(n, 2)).astype(rdt)[:n]
+    if n == 1:
+        z = np.array([x[0] + 0j], cdt)
+    else:
+        z = O.fft_ld((x[0::2] + 1j * x[1::2]).astype(np.complex128)).astype(cdt)
+    nout = n // 2 + 1 if half else n
+    want = O.unpack(z, n, half)
+    tz = torch.from_numpy(z).to(gpu)
+    out = torch.full((max(n, 2),), 43 + 21j, dtype=tcdt, device=gpu)
+    p = G.Plan(C.GFFT_DIT, n, precision=0 if prec == "f32" else 1, half=half)
+    p.execute(tz, out)
+    y = out.cpu().numpy()
+    assert O.rel_l2(y[:nout], want) <= _tol(prec, n)
+    if n > 1:
+        full = O.fft_ld(x.astype(np.complex128))[:nout]
+        assert O.rel_l2(y[:nout], full) <= 2 * _tol(prec, n)
+    assert np.all(y[nout:] == 43 + 21j)
+    # in place: F overwrites Z in a buffer of n slots
+    if n >= 2:
+        buf = torch.full((n,), 43 + 21j, dtype=tcdt, device=gpu)
+        buf[: n // 2] = tz
+        p.execute(buf, buf)
+        assert O.rel_l2(buf.cpu().numpy()[:nout], want) <= _tol(prec, n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [2, 4, 16, 1024, 1 << 15, 1 << 20])
+def test_c2r_round_trip(gpu, prec, n):
+    """irfft(rfft(x)) = n * x (unscaled), and c2r equals the oracle's inverse DFT
+    of the Hermitian-extended spectrum."""
+    import genfft_amd as G
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    rng = np.random.default_rng(77 + n)
+    x = rng.standard_normal((3, n)).astype(rdt)
+    tx = torch.from_numpy(x).to(gpu)
+    F = G.rfft(tx, half=True)
+    back = G.irfft(F, n).cpu().numpy()
+    assert O.rel_l2(back / n, x) <= 2 * _tol(prec, n)
+    Fn = F.cpu().numpy()
+    full = np.concatenate([Fn, np.conj(Fn[:, 1:n // 2][:, ::-1])], axis=1) if n > 2 else Fn
+    ref = O.fft_ld(full.astype(np.complex128), inv=True).real
+    assert O.rel_l2(back, ref) <= _tol(prec, n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 1024])
+def test_transform_real_and_interleave(gpu, golden, prec, n):
+    import genfft_amd as G
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    x = golden[f"real_{prec}_n{n}_in"]
+    x2 = golden[f"treal_{prec}_n{n}_in2"]
+    eps = 1e-8 + n * 1e-12 if prec == "f64" else 1e-5 + n * 1e-8
+    f = G.FFT(n, tcdt)
+    out = torch.empty(n, dtype=tcdt, device=gpu)
+    f.transform_real(out, torch.from_numpy(x).to(gpu))
+    assert np.max(np.abs(out.cpu().numpy() - golden[f"treal_{prec}_n{n}"])) <= eps
+    f.transform_interleave(out, torch.from_numpy(x).to(gpu), torch.from_numpy(x2).to(gpu))
+    z = out.cpu().numpy()
+    assert np.max(np.abs(z - golden[f"tinter_{prec}_n{n}"])) <= eps
+    # separate_2x_real_FFT recovers both spectra, aliasing out1 with the input
+    o1 = out.clone()
+    o2 = torch.empty_like(out)
+    G.separate_2x_real_FFT(o1, o2, o1, n)
+    a, b = O.separate_2x(z)
+    sep_tol = 1e-6 if prec == "f32" else 1e-14
+    assert np.max(np.abs(o1.cpu().numpy() - a)) <= sep_tol
+    assert np.max(np.abs(o2.cpu().numpy() - b)) <= sep_tol
+    assert np.max(np.abs(o1.cpu().numpy() - golden[f"treal_{prec}_n{n}"])) <= 2 * eps
+
+
+def test_real_full_size_properties(gpu):
+    """R2C at the benchmark shape (float32, N = 2^20, batch 256): Hermitian
+    consistency with the full spectrum, F_0 = sum x, spot rows vs the oracle."""
+    import genfft_amd as G
+    n, b = 1 << 20, 256
+    g = torch.Generator(device=gpu).manual_seed(1)
+    x = torch.randn(b, n, device=gpu, generator=g)
+    h = G.rfft(x, half=True)
+    s = x.to(torch.float64).sum(dim=1)
+    assert ((h[:, 0].real.to(torch.float64) - s).abs() / s.abs().clamp_min(1)).max().item() < 1e-4
+    for r in (0, 255):
+        ref = O.fft_ld(x[r].cpu().numpy().astype(np.complex128))[: n // 2 + 1]
+        assert O.rel_l2(h[r].cpu().numpy(), ref) <= O.tol_c64(n)
