@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c64_n2^20_b128", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one plain 1 GiB device copy (scale reference for PMC byte counters)")
     return ap.parse_args()
 
 
@@ -127,6 +129,14 @@ def main():
                       torch.randn(batch, n, device=dev, generator=gen, dtype=cdt.to_real()))
     y = torch.empty_like(x)
     plan = G.Plan(C.GFFT_C2C, n, precision=0 if prec == "c64" else 1, batch=batch)
+
+    if a.calib_copy:
+        # a plain copy of known size: reads and writes exactly 1 GiB
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(a.warmup):
         plan.execute(x, y)

@@ -154,6 +154,7 @@ struct TileArgs {
     int32_t n_tiles;      // tiles_inner * n_outer1 * n_outer2
     int32_t swap_in, swap_out;
     int32_t vec_in, vec_out;  // 16-byte vector access allowed (alignment proven on host)
+    int32_t xcd_remap;    // map blocks so consecutive tiles run on one XCD
     int32_t tw_mode;      // TW_NONE / TW_INNER (c * k) / TW_OUTER (o1 * k)
     uint32_t tw_mask;     // Ntw - 1
     int32_t tw_lo_bits;   // two-level split of the exponent
@@ -427,13 +428,25 @@ struct TileGeom {
     }
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
+// each XCD with its own L2.  This permutation hands XCD x a contiguous run
+// of tiles, so neighbouring tiles -- which share 128-byte lines when a tile
+// is narrower than a line -- are fetched into one L2 at about the same time.
+__device__ __forceinline__ int xcd_tile(int b, int grid)
+{
+    constexpr int XCDS = 8;
+    const int full = grid - grid % XCDS;
+    if (b >= full) return b;
+    return (b % XCDS) * (full / XCDS) + b / XCDS;
+}
+
 template <typename T, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
 __global__ void __launch_bounds__(NT) tile_fft_kernel(TileArgs a)
 {
     using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
     __shared__ cx<T> lds[G::ELEMS];
     const int tid = threadIdx.x;
-    const int tile = blockIdx.x;
+    const int tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int t2 = tile / a.tiles_inner;
     const int ti = tile - t2 * a.tiles_inner;
     const int o2 = t2 / a.n_outer1;

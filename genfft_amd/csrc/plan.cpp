@@ -282,6 +282,7 @@ struct gfft_plan_s {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int dir_mask = 3;  // bit0 forward allowed, bit1 inverse allowed
+    int xcd_remap = 1;
     size_t in_bytes = 0, out_bytes = 0;
     bool out_has_gaps = false;
     std::mutex host_mu;
@@ -782,6 +783,7 @@ int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
             a.swap_out = inv && l.last;
             a.vec_in = l.vin_ok && (((uintptr_t)ip & 15) == 0);
             a.vec_out = l.vout_ok && (((uintptr_t)op & 15) == 0);
+            a.xcd_remap = p->xcd_remap;
             void *args[] = {&a};
             HIP_TRY(hipLaunchKernel((const void *)l.k->fn, dim3((unsigned)l.grid), dim3(l.k->NT), args, 0, s));
             break;
@@ -907,6 +909,7 @@ int gfft_plan_create(gfft_plan *plan, const gfft_desc *desc)
     p->d = d;
     p->prec = d.precision;
     p->device = dev;
+    p->xcd_remap = (int)env_int("GFFT_XCD_REMAP", 1);
     if ((st = build_plan(p.get(), false))) return st;
     if ((st = validate(p.get()))) return st;
     if (p->scratch_bytes) {
