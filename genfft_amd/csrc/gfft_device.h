@@ -191,9 +191,16 @@ __device__ __forceinline__ cx<T> apply_tw(cx<T> v, cx<double> w)
 //   butterfly (s, j), j < L/R: inputs j + r*L/R, twiddle W_{NS*R}^(r*(j%NS)),
 //   outputs (j - j%NS)*R + j%NS + r*NS.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int M, int NT, class G, bool FASTJ, int NS, int R, bool LAST>
+// Four-step twiddle bases a persistent tile kernel loads ahead of time: the
+// last pass then needs W^(c*(j0 + i*NT/M + r*L/R)) = hj * hd^i * hr^r (FP64,
+// each base already the product of its two table entries).
+struct Tw4Pre {
+    cx<double> hj, hd, hr;
+};
+
+template <typename T, int L, int M, int NT, class G, bool FASTJ, int NS, int R, bool LAST, bool PRE>
 __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restrict__ tw, int tid,
-                                              const TileArgs &a, int s0, int o)
+                                              const TileArgs &a, int s0, int o, const Tw4Pre &pre)
 {
     constexpr int LR = L / R;
     constexpr int BF = M * LR;
@@ -239,7 +246,23 @@ __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restric
                 for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], w[i][r - 1]);
             }
             dft<R>(v[i]);
-            if constexpr (LAST) {
+            if constexpr (LAST && PRE) {
+                static_assert(!FASTJ && NT % M == 0, "prefetched twiddles need the s-fast mapping");
+                if (a.tw_mode != TW_NONE) {
+                    // i-th butterfly of this thread: j = j0 + i*NT/M
+                    cx<double> wi = pre.hj;
+                    const cx<double> wd = pre.hd, wr = pre.hr;
+                    for (int q = 0; q < i; q++) wi = cmul(wi, wd);
+                    cx<double> wk = wi;
+                    v[i][0] = apply_tw(v[i][0], wk);
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        wk = cmul(wk, wr);
+                        v[i][r] = apply_tw(v[i][r], wk);
+                    }
+                }
+            }
+            if constexpr (LAST && !PRE) {
                 // four-step twiddle W_Ntw^(c*k), k = j + r*L/R (here NS*R == L)
                 if (a.tw_mode != TW_NONE) {
                     const uint32_t c = a.tw_mode == TW_INNER ? (uint32_t)(s0 + s) : (uint32_t)o;
@@ -268,16 +291,32 @@ __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restric
     __syncthreads();
 }
 
-template <typename T, int L, int M, int NT, class G, bool FASTJ, int NS, int R, int... RS>
+template <typename T, int L, int M, int NT, class G, bool FASTJ, bool PRE, int NS, int R, int... RS>
 __device__ __forceinline__ void stockham_passes(cx<T> *lds, const cx<T> *__restrict__ tw, int tid,
-                                                const TileArgs &a, int s0, int o)
+                                                const TileArgs &a, int s0, int o, const Tw4Pre &pre)
 {
     constexpr bool last = sizeof...(RS) == 0;
-    stockham_pass<T, L, M, NT, G, FASTJ, NS, R, last>(lds, tw, tid, a, s0, o);
+    stockham_pass<T, L, M, NT, G, FASTJ, NS, R, last, PRE>(lds, tw, tid, a, s0, o, pre);
     if constexpr (!last) {
         constexpr int used = NS > 1 ? NS * (R - 1) : 0;
-        stockham_passes<T, L, M, NT, G, FASTJ, NS * R, RS...>(lds, tw + used, tid, a, s0, o);
+        stockham_passes<T, L, M, NT, G, FASTJ, PRE, NS * R, RS...>(lds, tw + used, tid, a, s0, o, pre);
     }
+}
+
+template <int R, int... RS>
+constexpr int last_radix()
+{
+    if constexpr (sizeof...(RS) == 0) return R;
+    else return last_radix<RS...>();
+}
+
+// number of Stockham pass twiddles of a radix sequence
+template <int NS, int R, int... RS>
+constexpr int pass_tw_count()
+{
+    constexpr int used = NS > 1 ? NS * (R - 1) : 0;
+    if constexpr (sizeof...(RS) == 0) return used;
+    else return used + pass_tw_count<NS * R, RS...>();
 }
 
 // ---------------------------------------------------------------------------
@@ -455,9 +494,167 @@ __global__ void __launch_bounds__(NT) tile_fft_kernel(TileArgs a)
     const int nvalid = min(M, a.n_inner - s0);
     load_tile<T, L, M, NT, G, INP>(lds, a, o2, o, s0, nvalid, tid);
     __syncthreads();
-    stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, 1, RS...>(
-        lds, reinterpret_cast<const cx<T> *>(a.pass_tw), tid, a, s0, o);
+    stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, false, 1, RS...>(
+        lds, reinterpret_cast<const cx<T> *>(a.pass_tw), tid, a, s0, o, Tw4Pre{});
     store_tile<T, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, tid);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent, software-pipelined form of tile_fft_kernel for tiles that fill
+// most of a CU's LDS (one or two workgroups per CU): while tile t runs its
+// Stockham passes and stores, the loads of the workgroup's next tile are
+// already in flight into registers.  Pass twiddles live in LDS (no global
+// loads inside the passes, so waiting on them never waits on the prefetch),
+// and the four-step twiddle bases of tile t are issued before the prefetch
+// of tile t+1 for the same reason (vector-memory counters drain in order).
+// ---------------------------------------------------------------------------
+template <typename T, int L, int M, int NT, class G, int INP>
+__device__ __forceinline__ bool fetch_tile(cx<T> (&reg)[M * L / NT], const TileArgs &a, int o2, int o, int s0,
+                                           int nvalid, int tid)
+{
+    constexpr int TOT = M * L;
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(a.in) + (int64_t)o * a.in_outer + (int64_t)o2 * a.in_outer2;
+    if constexpr (INP == PAT_ROW) in += (int64_t)s0 * a.in_inner;
+    else in += s0;
+    if constexpr (sizeof(T) == 4 && TOT % (2 * NT) == 0 && (INP == PAT_ROW ? L : M) % 2 == 0) {
+        if (a.vec_in && nvalid == M) {
+#pragma unroll
+            for (int it = 0; it < TOT / (2 * NT); it++) {
+                const int e = (tid + it * NT) * 2;
+                const float4 *p;
+                if constexpr (INP == PAT_ROW) p = reinterpret_cast<const float4 *>(in + (int64_t)(e / L) * a.in_inner + e % L);
+                else p = reinterpret_cast<const float4 *>(in + (int64_t)(e / M) * a.in_elem + e % M);
+                const float4 q = *p;
+                reg[2 * it] = {q.x, q.y};
+                reg[2 * it + 1] = {q.z, q.w};
+            }
+            return true;
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < TOT / NT; it++) {
+        const int e = tid + it * NT;
+        const int s = INP == PAT_ROW ? e / L : e % M;
+        const int n = INP == PAT_ROW ? e % L : e / M;
+        if (s < nvalid)
+            reg[it] = INP == PAT_ROW ? in[(int64_t)s * a.in_inner + n] : in[(int64_t)n * a.in_elem + s];
+    }
+    return false;
+}
+
+template <typename T, int L, int M, int NT, class G, int INP>
+__device__ __forceinline__ void put_tile(cx<T> *lds, const cx<T> (&reg)[M * L / NT], bool vec, const TileArgs &a,
+                                         int nvalid, int tid)
+{
+    constexpr int TOT = M * L;
+    const bool sw = a.swap_in != 0;
+    if (vec) {
+#pragma unroll
+        for (int it = 0; it < TOT / (2 * NT); it++) {
+            const int e = (tid + it * NT) * 2;
+            const cx<T> x0 = sw ? swap_ri(reg[2 * it]) : reg[2 * it];
+            const cx<T> x1 = sw ? swap_ri(reg[2 * it + 1]) : reg[2 * it + 1];
+            if constexpr (INP == PAT_ROW) {
+                lds[G::off(e / L, e % L)] = x0;
+                lds[G::off(e / L, e % L + 1)] = x1;
+            } else {
+                lds[G::off(e % M, e / M)] = x0;
+                lds[G::off(e % M + 1, e / M)] = x1;
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int it = 0; it < TOT / NT; it++) {
+        const int e = tid + it * NT;
+        const int s = INP == PAT_ROW ? e / L : e % M;
+        const int n = INP == PAT_ROW ? e % L : e / M;
+        if (s < nvalid) lds[G::off(s, n)] = sw ? swap_ri(reg[it]) : reg[it];
+    }
+}
+
+template <typename T, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
+__global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
+{
+    using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
+    constexpr int TWN = pass_tw_count<1, RS...>();
+    constexpr int E = M * L / NT;
+    constexpr int LAST_R = last_radix<RS...>();
+    __shared__ cx<T> lds[G::ELEMS];
+    __shared__ cx<T> twl[TWN > 0 ? TWN : 1];
+    const int tid = threadIdx.x;
+    {
+        const cx<T> *gtw = reinterpret_cast<const cx<T> *>(a.pass_tw);
+        for (int i = tid; i < TWN; i += NT) twl[i] = gtw[i];
+    }
+    const int grid = gridDim.x;
+    const int slot = a.xcd_remap ? xcd_tile(blockIdx.x, grid) : (int)blockIdx.x;
+    auto decode = [&](int t, int &o2, int &o, int &s0, int &nv) {
+        const int t2 = t / a.tiles_inner;
+        const int ti = t - t2 * a.tiles_inner;
+        o2 = t2 / a.n_outer1;
+        o = t2 - o2 * a.n_outer1;
+        s0 = ti * M;
+        nv = min(M, a.n_inner - s0);
+    };
+    cx<T> reg[E];
+    int tile = slot, o2 = 0, o = 0, s0 = 0, nvalid = 0;
+    bool vec = false;
+    if (tile < a.n_tiles) {
+        decode(tile, o2, o, s0, nvalid);
+        vec = fetch_tile<T, L, M, NT, G, INP>(reg, a, o2, o, s0, nvalid, tid);
+    }
+    while (tile < a.n_tiles) {
+        // Re-materialise the thread index every iteration: without this the
+        // compiler hoists every tid-derived LDS offset of every pass out of
+        // the loop and keeps them live, which costs ~100 VGPRs.
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        // four-step twiddle bases of this tile (s-fast mapping: s = t % M):
+        // issued first, so their latency hides behind the wait for the
+        // prefetched tile data that put_tile needs anyway
+        cx<double> hj{}, lj{}, hd{}, ld{}, hr{}, lr{};
+        const bool tw = LAYOUT == LAY_ELEM && NT % M == 0 && a.tw_mode != TW_NONE;
+        if (tw) {
+            const uint32_t c = a.tw_mode == TW_INNER ? (uint32_t)(s0 + t % M) : (uint32_t)o;
+            const cx<double> *lo = reinterpret_cast<const cx<double> *>(a.tw_lo);
+            const cx<double> *hi = reinterpret_cast<const cx<double> *>(a.tw_hi);
+            const uint32_t lm = (1u << a.tw_lo_bits) - 1u;
+            const uint32_t mj = (c * (uint32_t)(t / M)) & a.tw_mask;
+            const uint32_t md = (c * (uint32_t)(NT / M)) & a.tw_mask;
+            const uint32_t mr = (c * (uint32_t)(L / LAST_R)) & a.tw_mask;
+            hj = hi[mj >> a.tw_lo_bits];
+            lj = lo[mj & lm];
+            hd = hi[md >> a.tw_lo_bits];
+            ld = lo[md & lm];
+            hr = hi[mr >> a.tw_lo_bits];
+            lr = lo[mr & lm];
+        }
+        put_tile<T, L, M, NT, G, INP>(lds, reg, vec, a, nvalid, t);
+        Tw4Pre pre{};
+        if (tw) {
+            pre.hj = cmul(hj, lj);
+            pre.hd = cmul(hd, ld);
+            pre.hr = cmul(hr, lr);
+        }
+        __syncthreads();
+        // prefetch the next tile
+        const int next = tile + grid;
+        int no2 = 0, no = 0, ns0 = 0, nnv = 0;
+        if (next < a.n_tiles) {
+            decode(next, no2, no, ns0, nnv);
+            vec = fetch_tile<T, L, M, NT, G, INP>(reg, a, no2, no, ns0, nnv, t);
+        }
+        stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, (LAYOUT == LAY_ELEM && NT % M == 0), 1, RS...>(
+            lds, twl, t, a, s0, o, pre);
+        store_tile<T, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, t);
+        __syncthreads();
+        tile = next;
+        o2 = no2;
+        o = no;
+        s0 = ns0;
+        nvalid = nnv;
+    }
 }
 
 }  // namespace gfft

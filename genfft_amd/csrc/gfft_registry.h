@@ -18,6 +18,7 @@ struct KernelEntry {
     int nrad;
     int rad[8];
     int lds_bytes;
+    int persistent;  // 1: tile_fft_pipe_kernel (grid sized to the device, loops over tiles)
     void (*fn)(TileArgs);
 };
 
@@ -41,7 +42,23 @@ constexpr KernelEntry make_entry()
     const int r[] = {RS...};
     for (int i = 0; i < (int)sizeof...(RS); i++) e.rad[i] = r[i];
     e.lds_bytes = (int)(G::ELEMS * sizeof(cx<T>));
+    e.persistent = 0;
     e.fn = &tile_fft_kernel<T, L, M, NT, LAY, INP, OUTP, RS...>;
+    return e;
+}
+
+// Persistent, software-pipelined variant (same tile math, LDS-resident pass
+// twiddles, next tile prefetched into registers).
+template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
+constexpr KernelEntry make_pipe_entry()
+{
+    KernelEntry e = make_entry<T, L, M, NT, LAY, INP, OUTP, RS...>();
+    using G = TileGeom<L, M, LAY, INP, OUTP>;
+    constexpr int twn = pass_tw_count<1, RS...>();
+    static_assert((G::ELEMS + (twn > 0 ? twn : 1)) * sizeof(cx<T>) <= 160 * 1024, "tile + twiddles exceed LDS");
+    e.lds_bytes = (int)((G::ELEMS + (twn > 0 ? twn : 1)) * sizeof(cx<T>));
+    e.persistent = 1;
+    e.fn = &tile_fft_pipe_kernel<T, L, M, NT, LAY, INP, OUTP, RS...>;
     return e;
 }
 

@@ -55,6 +55,12 @@ const KernelEntry kTable[] = {
     make_entry<T, 1024, 8, 512, S_, R_, C_, 16, 16, 4>(),
     make_entry<T, 2048, 8, 1024, S_, R_, C_, 16, 16, 8>(),
     make_entry<T, 4096, 4, 1024, S_, R_, C_, 16, 16, 16>(),
+    // --- persistent pipelined forms, kept where they measured faster on
+    // MI355X (c64 2^20 x 128: column pass 0.565 -> 0.524 ms, transposing row
+    // pass 0.462 -> 0.392 ms); the L = 512 / 2048 and M = 8 forms fall to
+    // one workgroup per CU and lose to the plain kernels ---
+    make_pipe_entry<T, 1024, 16, 1024, E_, C_, C_, 16, 16, 4>(),
+    make_pipe_entry<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
 };
 }  // namespace
 
