@@ -29,7 +29,7 @@ from . import _capi as C
 from ._capi import GfftError
 
 __all__ = [
-    "Plan", "FFT", "FFTVert", "DIT", "FFT2D", "RealFFT", "RealFFT2D",
+    "Plan", "FFT", "FFTVert", "DIT", "FFT2D", "RealFFT", "RealFFT2D", "UnevenBatch",
     "separate_2x_real_FFT", "fft", "ifft", "rfft", "irfft", "fft2", "GfftError",
 ]
 
@@ -114,6 +114,57 @@ class Plan:
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             self._lib.gfft_plan_destroy(h)
+            self._h = None
+
+
+class UnevenBatch:
+    """Transforms of mixed power-of-two lengths in one plan (gfft_batch_plan).
+
+    Job j reads ``in[in_offsets[j] : in_offsets[j] + sizes[j]]`` and writes
+    ``out[out_offsets[j] : ...]`` (complex elements of flat buffers), forward or
+    unscaled inverse per ``inverse[j]``.  ``mode``: "graph" (capture once,
+    replay a HIP graph), "eager", or "per_job" (one plan launch per job).
+    """
+
+    MODES = {"graph": C.GFFT_BATCH_GRAPH, "eager": C.GFFT_BATCH_EAGER, "per_job": C.GFFT_BATCH_PER_JOB}
+
+    def __init__(self, sizes, in_offsets, out_offsets, inverse, dtype: torch.dtype = torch.complex64, device=None):
+        import numpy as np
+        n = len(sizes)
+        self._sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+        self._in = np.ascontiguousarray(in_offsets, dtype=np.int64)
+        self._out = np.ascontiguousarray(out_offsets, dtype=np.int64)
+        self._dir = np.ascontiguousarray(np.asarray(inverse, dtype=bool).astype(np.int32))
+        if not (len(self._in) == len(self._out) == len(self._dir) == n):
+            raise ValueError("sizes, offsets and directions must have the same length")
+        self._lib = C.lib()
+        self._h = ctypes.c_void_p()
+        p64 = ctypes.POINTER(ctypes.c_int64)
+        args = (ctypes.byref(self._h), _prec(dtype), n, self._sizes.ctypes.data_as(p64),
+                self._in.ctypes.data_as(p64), self._out.ctypes.data_as(p64),
+                self._dir.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        if device is not None:
+            with torch.cuda.device(device):
+                C.check(self._lib.gfft_batch_plan_create(*args))
+        else:
+            C.check(self._lib.gfft_batch_plan_create(*args))
+
+    def execute(self, inp: torch.Tensor, out: torch.Tensor, mode: str = "graph", stream=None) -> None:
+        dev = _check_gpu(inp, out)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        with torch.cuda.device(dev):
+            C.check(self._lib.gfft_batch_execute(self._h, inp.data_ptr(), out.data_ptr(), self.MODES[mode],
+                                                 s.cuda_stream))
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(1 << 20)
+        C.check(self._lib.gfft_batch_plan_describe(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.gfft_batch_plan_destroy(h)
             self._h = None
 
 

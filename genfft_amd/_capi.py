@@ -51,7 +51,15 @@ EXPORTS = (
     "gfft_error_string",
     "gfft_last_error",
     "gfft_version",
+    "gfft_batch_plan_create",
+    "gfft_batch_execute",
+    "gfft_batch_plan_describe",
+    "gfft_batch_plan_destroy",
 )
+
+GFFT_BATCH_GRAPH = 0
+GFFT_BATCH_EAGER = 1
+GFFT_BATCH_PER_JOB = 2
 
 
 class GfftDesc(ctypes.Structure):
@@ -123,6 +131,16 @@ def lib() -> ctypes.CDLL:
         L.gfft_last_error.restype = ctypes.c_char_p
         L.gfft_version.argtypes = []
         L.gfft_version.restype = i32
+        p64 = ctypes.POINTER(ctypes.c_int64)
+        L.gfft_batch_plan_create.argtypes = [ctypes.POINTER(vp), i32, i64, p64, p64, p64,
+                                             ctypes.POINTER(ctypes.c_int32)]
+        L.gfft_batch_plan_create.restype = i32
+        L.gfft_batch_execute.argtypes = [vp, vp, vp, i32, vp]
+        L.gfft_batch_execute.restype = i32
+        L.gfft_batch_plan_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.gfft_batch_plan_describe.restype = i32
+        L.gfft_batch_plan_destroy.argtypes = [vp]
+        L.gfft_batch_plan_destroy.restype = i32
         _lib = L
         return L
 

@@ -137,6 +137,13 @@ __device__ __forceinline__ void dft(cx<T> *v)
 // ---------------------------------------------------------------------------
 // Tile geometry and kernel arguments
 // ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ilog2c(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return l;
+}
+
 enum : int { PAT_ROW = 0, PAT_COL = 1 };
 enum : int { LAY_SEQ = 0, LAY_ELEM = 1 };
 enum : int { TW_NONE = 0, TW_INNER = 1, TW_OUTER = 2 };
@@ -158,6 +165,15 @@ struct TileArgs {
     int32_t tw_mode;      // TW_NONE / TW_INNER (c * k) / TW_OUTER (o1 * k)
     uint32_t tw_mask;     // Ntw - 1
     int32_t tw_lo_bits;   // two-level split of the exponent
+    // Optional job tables (uneven batches): when job_in / job_out is set, the
+    // transform o1 of this launch reads from a.in + job_in[job_first + o1]
+    // (writes to a.out + job_out[job_first + o1]) instead of o1 * in_outer,
+    // and job_inv[job_first + o1] selects the inverse where job_swap_* is set.
+    const int64_t *job_in;
+    const int64_t *job_out;
+    const uint8_t *job_inv;
+    int32_t job_first;
+    int32_t job_swap_in, job_swap_out;
     const void *pass_tw;  // Stockham pass twiddles, complex T
     const double *tw_lo;  // complex double [1 << tw_lo_bits]: W_Ntw^m
     const double *tw_hi;  // complex double [Ntw >> tw_lo_bits]: W_Ntw^(m << tw_lo_bits)
@@ -322,14 +338,31 @@ constexpr int pass_tw_count()
 // ---------------------------------------------------------------------------
 // HBM <-> LDS tile movement
 // ---------------------------------------------------------------------------
+// element offset of transform (o2, o1) and its direction swaps
+__device__ __forceinline__ int64_t in_base(const TileArgs &a, int o2, int o)
+{
+    return a.job_in ? a.job_in[a.job_first + o] : (int64_t)o * a.in_outer + (int64_t)o2 * a.in_outer2;
+}
+__device__ __forceinline__ int64_t out_base(const TileArgs &a, int o2, int o)
+{
+    return a.job_out ? a.job_out[a.job_first + o] : (int64_t)o * a.out_outer + (int64_t)o2 * a.out_outer2;
+}
+__device__ __forceinline__ bool swap_in_of(const TileArgs &a, int o)
+{
+    return a.job_swap_in ? a.job_inv[a.job_first + o] != 0 : a.swap_in != 0;
+}
+__device__ __forceinline__ bool swap_out_of(const TileArgs &a, int o)
+{
+    return a.job_swap_out ? a.job_inv[a.job_first + o] != 0 : a.swap_out != 0;
+}
+
 template <typename T, int L, int M, int NT, class G, int INP>
 __device__ __forceinline__ void load_tile(cx<T> *lds, const TileArgs &a, int o2, int o, int s0,
                                           int nvalid, int tid)
 {
     constexpr int TOT = M * L;
-    const bool sw = a.swap_in != 0;
-    const cx<T> *in = reinterpret_cast<const cx<T> *>(a.in) + (int64_t)o * a.in_outer +
-                      (int64_t)o2 * a.in_outer2;
+    const bool sw = swap_in_of(a, o);
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(a.in) + in_base(a, o2, o);
     if constexpr (INP == PAT_ROW) in += (int64_t)s0 * a.in_inner;
     else in += s0;
 
@@ -388,9 +421,8 @@ __device__ __forceinline__ void store_tile(const cx<T> *lds, const TileArgs &a, 
                                            int s0, int nvalid, int tid)
 {
     constexpr int TOT = M * L;
-    const bool sw = a.swap_out != 0;
-    cx<T> *out = reinterpret_cast<cx<T> *>(a.out) + (int64_t)o * a.out_outer +
-                 (int64_t)o2 * a.out_outer2;
+    const bool sw = swap_out_of(a, o);
+    cx<T> *out = reinterpret_cast<cx<T> *>(a.out) + out_base(a, o2, o);
     if constexpr (OUTP == PAT_ROW) out += (int64_t)s0 * a.out_inner;
     else out += s0;
 
@@ -513,7 +545,7 @@ __device__ __forceinline__ bool fetch_tile(cx<T> (&reg)[M * L / NT], const TileA
                                            int nvalid, int tid)
 {
     constexpr int TOT = M * L;
-    const cx<T> *in = reinterpret_cast<const cx<T> *>(a.in) + (int64_t)o * a.in_outer + (int64_t)o2 * a.in_outer2;
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(a.in) + in_base(a, o2, o);
     if constexpr (INP == PAT_ROW) in += (int64_t)s0 * a.in_inner;
     else in += s0;
     if constexpr (sizeof(T) == 4 && TOT % (2 * NT) == 0 && (INP == PAT_ROW ? L : M) % 2 == 0) {
@@ -544,10 +576,10 @@ __device__ __forceinline__ bool fetch_tile(cx<T> (&reg)[M * L / NT], const TileA
 
 template <typename T, int L, int M, int NT, class G, int INP>
 __device__ __forceinline__ void put_tile(cx<T> *lds, const cx<T> (&reg)[M * L / NT], bool vec, const TileArgs &a,
-                                         int nvalid, int tid)
+                                         int o, int nvalid, int tid)
 {
     constexpr int TOT = M * L;
-    const bool sw = a.swap_in != 0;
+    const bool sw = swap_in_of(a, o);
     if (vec) {
 #pragma unroll
         for (int it = 0; it < TOT / (2 * NT); it++) {
@@ -630,7 +662,7 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
             hr = hi[mr >> a.tw_lo_bits];
             lr = lo[mr & lm];
         }
-        put_tile<T, L, M, NT, G, INP>(lds, reg, vec, a, nvalid, t);
+        put_tile<T, L, M, NT, G, INP>(lds, reg, vec, a, o, nvalid, t);
         Tw4Pre pre{};
         if (tw) {
             pre.hj = cmul(hj, lj);

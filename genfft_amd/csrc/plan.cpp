@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "../../include/genfft_amd.h"
+#include "gfft_batch.h"
 #include "gfft_misc.h"
 #include "gfft_registry.h"
 
@@ -264,11 +265,32 @@ int64_t env_int(const char *name, int64_t dflt)
     return strtoll(v, nullptr, 10);
 }
 
+// Four-step split of a length 2^lg: column pass 2^l1, transposing row pass 2^l2.
+void four_step_split(int prec, int lg, int *l1, int *l2)
+{
+    int b = lg / 2, a = lg - b;  // keep the column length <= the row length
+    if (a > b) std::swap(a, b);
+    // complex64: the transposing row pass runs at copy speed at length 1024
+    // (pipelined kernel) while column passes are fastest short, so keep the
+    // row length at 1024 while the column length fits a tile (measured on
+    // MI355X for 2 GiB batches: 2^16 0.79 ms, 2^18 0.81 ms, 2^20 0.92 ms)
+    if (prec == 0 && lg - 10 >= 1 && ((int64_t)1 << (lg - 10)) <= max_len(prec, PAT_COL, PAT_COL)) {
+        b = 10;
+        a = lg - 10;
+    }
+    if (const int64_t f = env_int("GFFT_N1_LOG2", 0)) {  // experiment override of the split
+        a = (int)f;
+        b = lg - a;
+    }
+    *l1 = a;
+    *l2 = b;
+}
+
 // ---------------------------------------------------------------------------
 // Plan representation
 // ---------------------------------------------------------------------------
 enum BufId { B_IN = 0, B_OUT = 1, B_SCR = 2 };
-enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY };
+enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET };
 
 struct Launch {
     LaunchType type;
@@ -277,6 +299,7 @@ struct Launch {
     UnpackArgs ua{};
     HFillArgs ha{};
     CopyArgs ca{};
+    MixArgs mix{};
     int in_buf = B_IN, out_buf = B_OUT;
     int64_t in_off = 0, out_off = 0;  // bytes
     bool vin_ok = false, vout_ok = false;
@@ -413,21 +436,8 @@ struct Builder {
                         in_dist, 1, 0, 0, out_dist, 1, TW_NONE, 0, first, last);
         // four-step: n = N1 (column length) * N2 (row length)
         const int lg = ilog2(n);
-        int l2 = lg / 2;                       // row pass length 2^l2 .. keep N1 <= N2
-        int l1 = lg - l2;
-        if (l1 > l2) std::swap(l1, l2);
-        // complex64: the transposing row pass runs at copy speed at length
-        // 1024 (pipelined kernel) while column passes are fastest short, so
-        // keep the row length at 1024 while the column length fits a tile
-        // (measured: 2^16 0.79 ms, 2^18 0.81 ms, 2^20 0.92 ms for 2 GiB)
-        if (prec == 0 && lg - 10 >= 1 && ((int64_t)1 << (lg - 10)) <= max_len(prec, PAT_COL, PAT_COL)) {
-            l2 = 10;
-            l1 = lg - 10;
-        }
-        if (const int64_t f = env_int("GFFT_N1_LOG2", 0)) {  // experiment override of the split
-            l1 = (int)f;
-            l2 = lg - l1;
-        }
+        int l1, l2;
+        four_step_split(prec, lg, &l1, &l2);
         const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
         if (N1 > max_len(prec, PAT_COL, PAT_COL) || N2 > max_len(prec, PAT_ROW, PAT_COL))
             return fail(GFFT_ERR_UNSUPPORTED, "length %lld exceeds the two-pass limit", (long long)n);
@@ -746,15 +756,22 @@ int validate(const gfft_plan_s *p)
     return GFFT_OK;
 }
 
+int describe_launches(const std::vector<Launch> &launches, const std::string &head, char *buf, size_t len);
+
 int describe(const gfft_plan_s *p, char *buf, size_t len)
 {
-    std::string s;
-    char line[1024];
-    const char *bn[3] = {"in", "out", "scratch"};
+    char line[256];
     snprintf(line, sizeof(line), "plan in_bytes=%zu out_bytes=%zu scratch_bytes=%zu\n", p->in_bytes,
              p->out_bytes, p->scratch_bytes);
-    s += line;
-    for (const Launch &l : p->launches) {
+    return describe_launches(p->launches, line, buf, len);
+}
+
+int describe_launches(const std::vector<Launch> &launches, const std::string &head, char *buf, size_t len)
+{
+    std::string s = head;
+    char line[1024];
+    const char *bn[3] = {"in", "out", "scratch"};
+    for (const Launch &l : launches) {
         switch (l.type) {
         case LT_TILE: {
             const TileArgs &a = l.ta;
@@ -788,6 +805,12 @@ int describe(const gfft_plan_s *p, char *buf, size_t len)
             snprintf(line, sizeof(line), "copy batch=%lld in=%s+%lld out=%s+%lld\n", (long long)l.ca.batch,
                      bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off);
             break;
+        case LT_MIX:
+            snprintf(line, sizeof(line), "mix tasks=%d grid=%lld\n", l.mix.n_tasks, (long long)l.grid);
+            break;
+        case LT_MIX_RESET:
+            snprintf(line, sizeof(line), "mix_reset\n");
+            break;
         }
         s += line;
     }
@@ -798,13 +821,25 @@ int describe(const gfft_plan_s *p, char *buf, size_t len)
 
 // Executes the plan's launches in order on stream s.  With `events`, records
 // events[2i] / events[2i+1] on s around launch i (per-kernel timing).
+int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, const void *in, void *out,
+                 void *scratch, int direction, hipStream_t s, hipEvent_t *events = nullptr);
+
 int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
         hipEvent_t *events = nullptr)
 {
+    return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events);
+}
+
+int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, const void *in, void *out,
+                 void *scratch, int direction, hipStream_t s, hipEvent_t *events)
+{
+    struct {
+        int prec, xcd_remap;
+    } pp{prec, xcd_remap}, *p = &pp;
     const bool inv = direction == GFFT_INVERSE;
-    char *bufs[3] = {(char *)in, (char *)out, (char *)p->scratch};
+    char *bufs[3] = {(char *)in, (char *)out, (char *)scratch};
     int li = 0;
-    for (const Launch &L0 : p->launches) {
+    for (const Launch &L0 : launches) {
         Launch l = L0;
         if (events) HIP_TRY(hipEventRecord(events[2 * li], s));
         char *ip = bufs[l.in_buf] + l.in_off;
@@ -844,6 +879,17 @@ int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
             HIP_TRY(launch_copy(p->prec, a, s));
             break;
         }
+        case LT_MIX_RESET:
+            HIP_TRY(launch_mix_reset(l.mix.counter, s));
+            break;
+        case LT_MIX: {
+            MixArgs a = l.mix;
+            a.in = ip;
+            a.out = op;
+            a.vec = l.vin_ok && (((uintptr_t)ip & 15) == 0) && (((uintptr_t)op & 15) == 0);
+            HIP_TRY(launch_mix(p->prec, a, (int)l.grid, s));
+            break;
+        }
         }
         if (events) HIP_TRY(hipEventRecord(events[2 * li + 1], s));
         li++;
@@ -856,6 +902,180 @@ bool device_ok(int *dev)
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return false;
     return hipGetDevice(dev) == hipSuccess;
+}
+
+
+// ---------------------------------------------------------------------------
+// Uneven batches: transforms of mixed power-of-two lengths at arbitrary
+// offsets, each with its own direction.
+//   lengths 1 .. MixCfg::TOT   one persistent mixed-size launch (device work
+//                              queue), preceded by a one-thread counter reset;
+//   longer lengths            per length class, the single-pass or four-step
+//                              tile launches with job tables along the
+//                              transform index.
+// The whole list is captured once into a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+}  // namespace
+}  // namespace gfft
+
+struct gfft_batch_plan_s {
+    int prec = 0, device = 0, num_cus = 256, xcd_remap = 1;
+    int64_t count = 0;
+    std::vector<int64_t> sizes, in_off, out_off;
+    std::vector<int32_t> dirs;
+    int64_t *d_job_in = nullptr, *d_job_out = nullptr;
+    uint8_t *d_job_inv = nullptr;
+    gfft::MixTask *d_tasks = nullptr;
+    uint32_t *d_counter = nullptr;
+    std::vector<gfft::Launch> launches;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    bool offsets_even = true;
+    std::mutex mu;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *g_in = nullptr;
+    void *g_out = nullptr;
+    hipStream_t cap_stream = nullptr;
+    std::map<int64_t, gfft_plan> per_job;  // one single-transform plan per length (reference loop)
+};
+
+namespace gfft {
+namespace {
+
+int build_batch(gfft_batch_plan_s *p)
+{
+    const int prec = p->prec;
+    const size_t cs = prec ? 16 : 8;
+    const int64_t n = p->count;
+    // sort jobs by length (stable: original order inside a class)
+    std::vector<int64_t> order(n);
+    for (int64_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return p->sizes[a] < p->sizes[b]; });
+    std::vector<int64_t> jin(n), jout(n);
+    std::vector<uint8_t> jinv(n);
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t j = order[k];
+        jin[k] = p->in_off[j];
+        jout[k] = p->out_off[j];
+        jinv[k] = p->dirs[j] == GFFT_INVERSE ? 1 : 0;
+        p->offsets_even = p->offsets_even && (jin[k] % 2 == 0) && (jout[k] % 2 == 0);
+    }
+    int st;
+    if ((st = upload((void **)&p->d_job_in, jin.data(), n * sizeof(int64_t)))) return st;
+    if ((st = upload((void **)&p->d_job_out, jout.data(), n * sizeof(int64_t)))) return st;
+    if ((st = upload((void **)&p->d_job_inv, jinv.data(), n))) return st;
+    HIP_TRY(hipMalloc((void **)&p->d_counter, 256));
+    HIP_TRY(hipMemset(p->d_counter, 0, 256));
+
+    // classes
+    const int mix_max = prec ? MixCfg<double>::MAXLG : MixCfg<float>::MAXLG;
+    const int mix_tot = prec ? MixCfg<double>::TOT : MixCfg<float>::TOT;
+    std::vector<MixTask> tasks;
+    struct Cls {
+        int lg;
+        int64_t first, count;
+    };
+    std::vector<Cls> big;
+    for (int64_t k = 0; k < n;) {
+        int64_t e = k;
+        while (e < n && p->sizes[order[e]] == p->sizes[order[k]]) e++;
+        const int lg = ilog2(p->sizes[order[k]]);
+        if (lg <= mix_max) {
+            const int m = lg == 0 ? 64 : mix_m(mix_tot, 1 << lg);
+            for (int64_t f = k; f < e; f += m) tasks.push_back(MixTask{lg, (int32_t)f, (int32_t)std::min<int64_t>(m, e - f), 0});
+        } else {
+            big.push_back(Cls{lg, k, e - k});
+        }
+        k = e;
+    }
+    // largest tiles first: better balance for the dynamic queue
+    std::stable_sort(tasks.begin(), tasks.end(), [](const MixTask &a, const MixTask &b) { return a.lg > b.lg; });
+
+    if (!tasks.empty()) {
+        if ((st = upload((void **)&p->d_tasks, tasks.data(), tasks.size() * sizeof(MixTask)))) return st;
+        Launch r;
+        r.type = LT_MIX_RESET;
+        r.mix.counter = p->d_counter;
+        p->launches.push_back(r);
+        Launch l;
+        l.type = LT_MIX;
+        l.mix.job_in = p->d_job_in;
+        l.mix.job_out = p->d_job_out;
+        l.mix.job_inv = p->d_job_inv;
+        l.mix.tasks = p->d_tasks;
+        l.mix.counter = p->d_counter;
+        l.mix.n_tasks = (int32_t)tasks.size();
+        for (int lg = 1; lg <= mix_max; lg++) {
+            KernelEntry k{};
+            k.prec = prec;
+            k.L = 1 << lg;
+            k.nrad = mix_radices(prec, lg, k.rad);
+            if ((st = get_pass_twiddles(p->device, k, &l.mix.pass_tw[lg], false))) return st;
+        }
+        l.vin_ok = p->offsets_even && prec == 0;
+        l.grid = std::min<int64_t>((int64_t)tasks.size(), mix_grid(prec, p->num_cus));
+        p->launches.push_back(l);
+    }
+
+    // length classes above the mixed kernel: job tables along the transform index
+    gfft_plan_s tmp;
+    tmp.prec = prec;
+    tmp.device = p->device;
+    tmp.num_cus = p->num_cus;
+    Builder b{&tmp, prec, cs, false};
+    const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
+    auto set_jobs = [&](int64_t first, bool in_side, bool out_side) {
+        TileArgs &a = tmp.launches.back().ta;
+        if (in_side) {
+            a.job_in = p->d_job_in;
+            a.job_swap_in = 1;
+        }
+        if (out_side) {
+            a.job_out = p->d_job_out;
+            a.job_swap_out = 1;
+        }
+        a.job_inv = p->d_job_inv;
+        a.job_first = (int32_t)first;
+        Launch &l = tmp.launches.back();
+        if (in_side) l.vin_ok = l.vin_ok && p->offsets_even;
+        if (out_side) l.vout_ok = l.vout_ok && p->offsets_even;
+    };
+    for (const Cls &c : big) {
+        const int64_t L = (int64_t)1 << c.lg;
+        if (L <= maxrow) {
+            if ((st = b.tile((int)L, PAT_ROW, PAT_ROW, 1, B_IN, 0, B_OUT, 0, 1, c.count, 1, 0, 0, 0, 1, 0, 0, 0, 1,
+                             TW_NONE, 0, false, false)))
+                return st;
+            set_jobs(c.first, true, true);
+            continue;
+        }
+        int l1, l2;
+        four_step_split(prec, c.lg, &l1, &l2);
+        const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(c.count, ((int64_t)1 << 30) / (L * (int64_t)cs)));
+        b.need_scratch(chunk * L);
+        for (int64_t b0 = 0; b0 < c.count; b0 += chunk) {
+            const int64_t cb = std::min(chunk, c.count - b0);
+            if ((st = b.tile((int)N1, PAT_COL, PAT_COL, 0, B_IN, 0, B_SCR, 0, N2, cb, 1, 0, 0, 1, N2, 0, L, 1, N2,
+                             TW_INNER, L, false, false)))
+                return st;
+            set_jobs(c.first + b0, true, false);
+            if ((st = b.tile((int)N2, PAT_ROW, PAT_COL, 0, B_SCR, 0, B_OUT, 0, N1, cb, 1, 0, L, N2, 1, 0, 0, 1, N1,
+                             TW_NONE, 0, false, false)))
+                return st;
+            set_jobs(c.first + b0, false, true);
+        }
+    }
+    for (auto &l : tmp.launches) p->launches.push_back(l);
+    p->scratch_bytes = (size_t)b.scratch_elems * cs;
+    if (p->scratch_bytes) HIP_TRY(hipMalloc(&p->scratch, p->scratch_bytes));
+    return GFFT_OK;
+}
+
+int enqueue_batch(gfft_batch_plan_s *p, const void *in, void *out, hipStream_t s)
+{
+    return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, s);
 }
 
 }  // namespace
@@ -1050,6 +1270,124 @@ int gfft_interleave_real(int precision, void *out, const void *in1, const void *
     InterleaveArgs a{in1, in2, out, n, batch};
     HIP_TRY(launch_interleave(precision, a, (hipStream_t)stream));
     return GFFT_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int gfft_batch_plan_create(gfft_batch_plan *plan, int precision, int64_t count, const int64_t *sizes,
+                           const int64_t *in_offsets, const int64_t *out_offsets, const int32_t *directions)
+{
+    if (!plan || !sizes || !in_offsets || !out_offsets || !directions || count < 1)
+        return fail(GFFT_ERR_INVALID_ARG, "null argument or empty batch");
+    *plan = nullptr;
+    if (precision != GFFT_F32 && precision != GFFT_F64) return fail(GFFT_ERR_INVALID_ARG, "bad precision");
+    for (int64_t j = 0; j < count; j++) {
+        if (!is_pow2(sizes[j]) || sizes[j] > ((int64_t)1 << 23))
+            return fail(GFFT_ERR_UNSUPPORTED, "job %lld: size %lld is not a power of two in 1..2^23", (long long)j,
+                        (long long)sizes[j]);
+        if (in_offsets[j] < 0 || out_offsets[j] < 0)
+            return fail(GFFT_ERR_INVALID_ARG, "job %lld: negative offset", (long long)j);
+        if (directions[j] != GFFT_FORWARD && directions[j] != GFFT_INVERSE)
+            return fail(GFFT_ERR_INVALID_ARG, "job %lld: bad direction", (long long)j);
+    }
+    int dev = 0;
+    if (!device_ok(&dev)) return fail(GFFT_ERR_NO_DEVICE, "no HIP device available");
+    std::unique_ptr<gfft_batch_plan_s> p(new gfft_batch_plan_s());
+    p->prec = precision;
+    p->device = dev;
+    p->count = count;
+    p->sizes.assign(sizes, sizes + count);
+    p->in_off.assign(in_offsets, in_offsets + count);
+    p->out_off.assign(out_offsets, out_offsets + count);
+    p->dirs.assign(directions, directions + count);
+    p->xcd_remap = (int)env_int("GFFT_XCD_REMAP", 1);
+    if (hipDeviceGetAttribute(&p->num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || p->num_cus < 1)
+        p->num_cus = 256;
+    int st = build_batch(p.get());
+    if (st) {
+        gfft_batch_plan_destroy(p.release());
+        return st;
+    }
+    *plan = p.release();
+    return GFFT_OK;
+}
+
+int gfft_batch_plan_destroy(gfft_batch_plan plan)
+{
+    if (!plan) return GFFT_OK;
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    if (plan->cap_stream) (void)hipStreamDestroy(plan->cap_stream);
+    for (auto &kv : plan->per_job) gfft_plan_destroy(kv.second);
+    for (void *ptr : {(void *)plan->d_job_in, (void *)plan->d_job_out, (void *)plan->d_job_inv,
+                      (void *)plan->d_tasks, (void *)plan->d_counter, plan->scratch})
+        if (ptr) (void)hipFree(ptr);
+    delete plan;
+    return GFFT_OK;
+}
+
+int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode, void *stream)
+{
+    if (!plan || !in || !out) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    const hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(plan->mu);
+    if (mode == GFFT_BATCH_EAGER) return enqueue_batch(plan, in, out, s);
+    if (mode == GFFT_BATCH_PER_JOB) {
+        // the plain path: one single-transform plan launch per job
+        for (int64_t j = 0; j < plan->count; j++) {
+            gfft_plan &pj = plan->per_job[plan->sizes[j]];
+            if (!pj) {
+                gfft_desc d{};
+                d.kind = GFFT_C2C;
+                d.precision = plan->prec;
+                d.n = plan->sizes[j];
+                d.batch = 1;
+                int st = gfft_plan_create(&pj, &d);
+                if (st) return st;
+            }
+            const size_t cs = plan->prec ? 16 : 8;
+            int st = gfft_execute(pj, (const char *)in + plan->in_off[j] * cs, (char *)out + plan->out_off[j] * cs,
+                                  plan->dirs[j], stream);
+            if (st) return st;
+        }
+        return GFFT_OK;
+    }
+    if (mode != GFFT_BATCH_GRAPH) return fail(GFFT_ERR_INVALID_ARG, "unknown batch mode %d", mode);
+    if (!plan->exec || plan->g_in != in || plan->g_out != out) {
+        if (plan->exec) {
+            (void)hipGraphExecDestroy(plan->exec);
+            (void)hipGraphDestroy(plan->graph);
+            plan->exec = nullptr;
+            plan->graph = nullptr;
+        }
+        if (!plan->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
+        int st = enqueue_batch(plan, in, out, plan->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(plan->cap_stream, &g);
+        if (st) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return fail(GFFT_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        plan->graph = g;
+        HIP_TRY(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+        plan->g_in = in;
+        plan->g_out = out;
+    }
+    HIP_TRY(hipGraphLaunch(plan->exec, s));
+    return GFFT_OK;
+}
+
+int gfft_batch_plan_describe(gfft_batch_plan plan, char *buf, size_t len)
+{
+    if (!plan || !buf || !len) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    char head[256];
+    snprintf(head, sizeof(head), "batch jobs=%lld scratch_bytes=%zu offsets_even=%d\n", (long long)plan->count,
+             plan->scratch_bytes, (int)plan->offsets_even);
+    return describe_launches(plan->launches, head, buf, len);
 }
 
 }  // extern "C"

@@ -146,6 +146,28 @@ GFFT_API int gfft_separate_2x_real(int precision, void *out1, void *out2, const 
 GFFT_API int gfft_interleave_real(int precision, void *out, const void *in1, const void *in2, int64_t n,
                          int64_t batch, void *stream);
 
+/* ---- uneven batches (no reference counterpart: genFFT runs one transform
+ * per call) ------------------------------------------------------------------
+ * `count` transforms of power-of-two lengths sizes[j] (1..2^23), job j reading
+ * in + in_offsets[j] and writing out + out_offsets[j] (complex elements),
+ * forward or (unscaled) inverse per directions[j].  Lengths up to 4096 (c64)
+ * / 2048 (c128) run in one persistent launch that drains a device-side task
+ * queue; longer lengths run as per-length batched launches. */
+typedef struct gfft_batch_plan_s *gfft_batch_plan;
+
+#define GFFT_BATCH_GRAPH 0    /* capture the launches once into a HIP graph and replay it */
+#define GFFT_BATCH_EAGER 1    /* enqueue the launches directly */
+#define GFFT_BATCH_PER_JOB 2  /* the plain path: one single-transform plan launch per job */
+
+GFFT_API int gfft_batch_plan_create(gfft_batch_plan *plan, int precision, int64_t count, const int64_t *sizes,
+                                    const int64_t *in_offsets, const int64_t *out_offsets,
+                                    const int32_t *directions);
+/* Device buffers; asynchronous on `stream`.  The graph is re-captured when in
+ * or out change.  Calls on one plan may not overlap in time. */
+GFFT_API int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode, void *stream);
+GFFT_API int gfft_batch_plan_describe(gfft_batch_plan plan, char *buf, size_t len);
+GFFT_API int gfft_batch_plan_destroy(gfft_batch_plan plan);
+
 /* 1 if p is device (or managed) memory visible to the current device. */
 GFFT_API int gfft_is_device_pointer(const void *p);
 
