@@ -31,6 +31,7 @@ from ._capi import GfftError
 __all__ = [
     "Plan", "FFT", "FFTVert", "DIT", "FFT2D", "RealFFT", "RealFFT2D", "UnevenBatch",
     "separate_2x_real_FFT", "fft", "ifft", "rfft", "irfft", "fft2", "GfftError",
+    "autotune", "save_tuning", "load_tuning",
 ]
 
 _COMPLEX = {torch.complex64: C.GFFT_F32, torch.complex128: C.GFFT_F64}
@@ -166,6 +167,27 @@ class UnevenBatch:
         if h is not None and h.value:
             self._lib.gfft_batch_plan_destroy(h)
             self._h = None
+
+
+def autotune(n: int, batch: int, dtype: torch.dtype = torch.complex64, reps: int = 10) -> dict:
+    """Times the candidate kernel choices for a batched C2C of length n on the
+    current device and keeps the fastest for later plans (gfft_autotune).
+    Plans cached before the call keep their old choice; returns the report."""
+    import json
+    buf = ctypes.create_string_buffer(1 << 20)
+    C.check(C.lib().gfft_autotune(_prec(dtype), n, batch, reps, buf, len(buf)))
+    with _cache_lock:
+        for k in [k for k in _cache if k[0] == C.GFFT_C2C and k[1] == n]:
+            del _cache[k]
+    return json.loads(buf.value.decode())
+
+
+def save_tuning(path: str) -> None:
+    C.check(C.lib().gfft_tuning_save(path.encode()))
+
+
+def load_tuning(path: str) -> None:
+    C.check(C.lib().gfft_tuning_load(path.encode()))
 
 
 # Plan cache: the analogue of the reference's per-(N, T) singleton

@@ -55,6 +55,10 @@ EXPORTS = (
     "gfft_batch_execute",
     "gfft_batch_plan_describe",
     "gfft_batch_plan_destroy",
+    "gfft_autotune",
+    "gfft_tuning_save",
+    "gfft_tuning_load",
+    "gfft_tuning_clear",
 )
 
 GFFT_BATCH_GRAPH = 0
@@ -141,6 +145,14 @@ def lib() -> ctypes.CDLL:
         L.gfft_batch_plan_describe.restype = i32
         L.gfft_batch_plan_destroy.argtypes = [vp]
         L.gfft_batch_plan_destroy.restype = i32
+        L.gfft_autotune.argtypes = [i32, i64, i64, i32, ctypes.c_char_p, ctypes.c_size_t]
+        L.gfft_autotune.restype = i32
+        L.gfft_tuning_save.argtypes = [ctypes.c_char_p]
+        L.gfft_tuning_save.restype = i32
+        L.gfft_tuning_load.argtypes = [ctypes.c_char_p]
+        L.gfft_tuning_load.restype = i32
+        L.gfft_tuning_clear.argtypes = []
+        L.gfft_tuning_clear.restype = i32
         _lib = L
         return L
 

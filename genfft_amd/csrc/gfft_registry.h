@@ -19,6 +19,7 @@ struct KernelEntry {
     int rad[8];
     int lds_bytes;
     int persistent;  // 1: tile_fft_pipe_kernel (grid sized to the device, loops over tiles)
+    int tune_only;   // 1: an alternative (radix order / tile width / threads) only the autotuner selects
     void (*fn)(TileArgs);
 };
 
@@ -43,7 +44,16 @@ constexpr KernelEntry make_entry()
     for (int i = 0; i < (int)sizeof...(RS); i++) e.rad[i] = r[i];
     e.lds_bytes = (int)(G::ELEMS * sizeof(cx<T>));
     e.persistent = 0;
+    e.tune_only = 0;
     e.fn = &tile_fft_kernel<T, L, M, NT, LAY, INP, OUTP, RS...>;
+    return e;
+}
+
+template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
+constexpr KernelEntry make_variant()
+{
+    KernelEntry e = make_entry<T, L, M, NT, LAY, INP, OUTP, RS...>();
+    e.tune_only = 1;
     return e;
 }
 

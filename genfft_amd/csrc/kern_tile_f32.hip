@@ -61,6 +61,20 @@ const KernelEntry kTable[] = {
     // one workgroup per CU and lose to the plain kernels ---
     make_pipe_entry<T, 1024, 16, 1024, E_, C_, C_, 16, 16, 4>(),
     make_pipe_entry<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
+    // --- autotuner alternatives: radix order, tile width, threads per tile ---
+    make_variant<T, 256, 32, 512, S_, R_, R_, 16, 16>(),
+    make_variant<T, 512, 8, 256, S_, R_, R_, 16, 8, 4>(),
+    make_variant<T, 512, 16, 512, S_, R_, R_, 8, 8, 8>(),
+    make_variant<T, 1024, 4, 256, S_, R_, R_, 4, 16, 16>(),
+    make_variant<T, 1024, 4, 256, S_, R_, R_, 8, 8, 16>(),
+    make_variant<T, 1024, 8, 512, S_, R_, R_, 16, 16, 4>(),
+    make_variant<T, 1024, 2, 128, S_, R_, R_, 16, 16, 4>(),
+    make_variant<T, 2048, 2, 256, S_, R_, R_, 8, 16, 16>(),
+    make_variant<T, 2048, 4, 512, S_, R_, R_, 16, 16, 8>(),
+    make_variant<T, 4096, 2, 512, S_, R_, R_, 16, 16, 16>(),
+    make_variant<T, 4096, 1, 512, S_, R_, R_, 8, 8, 8, 8>(),
+    make_variant<T, 8192, 1, 512, S_, R_, R_, 16, 16, 16, 2>(),
+    make_variant<T, 16384, 1, 1024, S_, R_, R_, 16, 16, 8, 8>(),
 };
 }  // namespace
 

@@ -46,6 +46,11 @@ const KernelEntry kTable[] = {
     make_entry<T, 1024, 8, 512, S_, R_, C_, 16, 16, 4>(),
     make_entry<T, 2048, 4, 512, S_, R_, C_, 16, 16, 8>(),
     make_entry<T, 4096, 2, 512, S_, R_, C_, 16, 16, 16>(),
+    // --- autotuner alternatives ---
+    make_variant<T, 1024, 4, 256, S_, R_, R_, 16, 16, 4>(),
+    make_variant<T, 1024, 2, 128, S_, R_, R_, 4, 16, 16>(),
+    make_variant<T, 2048, 2, 256, S_, R_, R_, 16, 16, 8>(),
+    make_variant<T, 4096, 1, 512, S_, R_, R_, 8, 8, 8, 8>(),
 };
 }  // namespace
 

@@ -224,16 +224,27 @@ int get_two_level(int dev, int64_t N, TwTable *t, bool dry)
 // ---------------------------------------------------------------------------
 int64_t env_int(const char *name, int64_t dflt);
 
-// Picks the tile kernel for (precision, L, patterns): the widest tile (M),
-// its persistent pipelined form when one is compiled (GFFT_PIPE=0 disables),
-// or the requested M.
-const KernelEntry *find_tile(int prec, int L, int inp, int outp, int prefer_m = 0)
+// Per-plan kernel choices: the planner's defaults, an autotuned entry from the
+// tuning table, or GFFT_* environment overrides (experiments).
+struct PlanOpts {
+    int n1_log2 = 0;       // four-step column length 2^n1_log2 (0: default split)
+    int col_m = 0;         // preferred column-tile width (0: widest)
+    int row_m = 0;         // preferred transposing-row tile width (0: widest)
+    int pipe_mask = 3;     // pipelined kernels allowed: bit 0 column tiles, bit 1 transposing rows
+    int row_variant = -1;  // registry index of the single-pass row kernel (-1: default)
+};
+
+// Picks the tile kernel for (precision, L, patterns): the registry entry
+// `variant` when it matches, else the widest tile (or the requested M) in its
+// persistent pipelined form when one is compiled and allowed by pipe_mask.
+const KernelEntry *find_tile(int prec, int L, int inp, int outp, int prefer_m = 0, int pipe_mask = 3,
+                             int variant = -1)
 {
     int cnt = 0;
     const KernelEntry *tab = prec ? tile_kernels_f64(&cnt) : tile_kernels_f32(&cnt);
-    // GFFT_PIPE bit 0: column tiles, bit 1: transposing row tiles
-    const int64_t mask = env_int("GFFT_PIPE", 3);
-    const bool pipe = (inp == PAT_COL ? (mask & 1) : (mask & 2)) != 0;
+    if (variant >= 0 && variant < cnt && tab[variant].L == L && tab[variant].inp == inp && tab[variant].outp == outp)
+        return &tab[variant];
+    const bool pipe = (inp == PAT_COL ? (pipe_mask & 1) : (pipe_mask & 2)) != 0;
     const KernelEntry *best = nullptr;
     auto better = [&](const KernelEntry &e) {
         if (!best) return true;
@@ -243,7 +254,7 @@ const KernelEntry *find_tile(int prec, int L, int inp, int outp, int prefer_m = 
     };
     for (int i = 0; i < cnt; i++) {
         const KernelEntry &e = tab[i];
-        if (e.L != L || e.inp != inp || e.outp != outp) continue;
+        if (e.L != L || e.inp != inp || e.outp != outp || e.tune_only) continue;
         if (better(e)) best = &e;
     }
     return best;
@@ -266,7 +277,7 @@ int64_t env_int(const char *name, int64_t dflt)
 }
 
 // Four-step split of a length 2^lg: column pass 2^l1, transposing row pass 2^l2.
-void four_step_split(int prec, int lg, int *l1, int *l2)
+void four_step_split(int prec, int lg, int *l1, int *l2, int n1_override = 0)
 {
     int b = lg / 2, a = lg - b;  // keep the column length <= the row length
     if (a > b) std::swap(a, b);
@@ -278,12 +289,45 @@ void four_step_split(int prec, int lg, int *l1, int *l2)
         b = 10;
         a = lg - 10;
     }
-    if (const int64_t f = env_int("GFFT_N1_LOG2", 0)) {  // experiment override of the split
-        a = (int)f;
+    if (n1_override > 0 && n1_override < lg) {
+        a = n1_override;
         b = lg - a;
     }
     *l1 = a;
     *l2 = b;
+}
+
+// ---------------------------------------------------------------------------
+// Tuning table: (precision, C2C length) -> kernel choices found by
+// gfft_autotune, optionally loaded from the JSON file named by
+// GFFT_TUNING_FILE at the first plan creation.
+// ---------------------------------------------------------------------------
+std::mutex g_tune_mu;
+std::map<std::pair<int, int64_t>, PlanOpts> g_tune;
+bool g_tune_loaded = false;
+
+int tuning_load_locked(const char *path);
+
+PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
+{
+    if (forced) return *forced;
+    PlanOpts o;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (!g_tune_loaded) {
+            g_tune_loaded = true;
+            if (const char *f = getenv("GFFT_TUNING_FILE"))
+                if (*f) (void)tuning_load_locked(f);
+        }
+        auto it = g_tune.find({prec, n});
+        if (n > 0 && it != g_tune.end()) o = it->second;
+    }
+    // environment overrides (experiments)
+    if (getenv("GFFT_N1_LOG2")) o.n1_log2 = (int)env_int("GFFT_N1_LOG2", 0);
+    if (getenv("GFFT_COL_M")) o.col_m = (int)env_int("GFFT_COL_M", 0);
+    if (getenv("GFFT_ROW_M")) o.row_m = (int)env_int("GFFT_ROW_M", 0);
+    if (getenv("GFFT_PIPE")) o.pipe_mask = (int)env_int("GFFT_PIPE", 3);
+    return o;
 }
 
 // ---------------------------------------------------------------------------
@@ -334,6 +378,8 @@ struct Builder {
     int prec;
     size_t csize;  // bytes of one complex element
     bool dry;      // plan without a device: no table uploads
+    const PlanOpts *forced = nullptr;  // autotuner candidate (overrides table and environment)
+    PlanOpts opts = resolve_opts(prec, 0, forced);
     int64_t scratch_elems = 0;  // complex elements of scratch needed
 
     void need_scratch(int64_t elems) { scratch_elems = std::max(scratch_elems, elems); }
@@ -343,9 +389,9 @@ struct Builder {
              int64_t out_off_el, int64_t n_inner, int64_t n_outer1, int64_t n_outer2,
              int64_t in_outer2, int64_t in_outer, int64_t in_inner, int64_t in_elem,
              int64_t out_outer2, int64_t out_outer, int64_t out_inner, int64_t out_elem,
-             int tw_mode, int64_t tw_n, bool first, bool last)
+             int tw_mode, int64_t tw_n, bool first, bool last, int variant = -1)
     {
-        const KernelEntry *k = find_tile(prec, L, inp, outp, prefer_m);
+        const KernelEntry *k = find_tile(prec, L, inp, outp, prefer_m, opts.pipe_mask, variant);
         if (!k) return fail(GFFT_ERR_UNSUPPORTED, "no tile kernel for L=%d pattern %d->%d", L, inp, outp);
         Launch l;
         l.type = LT_TILE;
@@ -431,13 +477,20 @@ struct Builder {
             return GFFT_OK;
         }
         const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
+        const PlanOpts saved = opts;
+        opts = resolve_opts(prec, n, forced);
+        struct Restore {
+            PlanOpts &dst;
+            PlanOpts v;
+            ~Restore() { dst = v; }
+        } restore{opts, saved};
         if (n <= maxrow)
             return tile((int)n, PAT_ROW, PAT_ROW, 0, in_buf, in_off, out_buf, out_off, batch, 1, 1, 0, 0,
-                        in_dist, 1, 0, 0, out_dist, 1, TW_NONE, 0, first, last);
+                        in_dist, 1, 0, 0, out_dist, 1, TW_NONE, 0, first, last, opts.row_variant);
         // four-step: n = N1 (column length) * N2 (row length)
         const int lg = ilog2(n);
         int l1, l2;
-        four_step_split(prec, lg, &l1, &l2);
+        four_step_split(prec, lg, &l1, &l2, opts.n1_log2);
         const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
         if (N1 > max_len(prec, PAT_COL, PAT_COL) || N2 > max_len(prec, PAT_ROW, PAT_COL))
             return fail(GFFT_ERR_UNSUPPORTED, "length %lld exceeds the two-pass limit", (long long)n);
@@ -449,7 +502,7 @@ struct Builder {
         int64_t chunk = std::max<int64_t>(1, chunk_bytes / (n * (int64_t)csize));
         chunk = std::min(chunk, batch);
         need_scratch(chunk * n);
-        const int pm1 = (int)env_int("GFFT_COL_M", 0), pm2 = (int)env_int("GFFT_ROW_M", 0);
+        const int pm1 = opts.col_m, pm2 = opts.row_m;
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
             const int64_t cb = std::min(chunk, batch - b0);
             int st = tile((int)N1, PAT_COL, PAT_COL, pm1, in_buf, in_off + b0 * in_dist, B_SCR, 0, N2, cb, 1, 0,
@@ -527,10 +580,10 @@ struct Builder {
 
 int64_t dflt(int64_t v, int64_t d) { return v ? v : d; }
 
-int build_plan(gfft_plan_s *p, bool dry)
+int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
 {
     const gfft_desc &d = p->d;
-    Builder b{p, p->prec, (size_t)(p->prec ? 16 : 8), dry};
+    Builder b{p, p->prec, (size_t)(p->prec ? 16 : 8), dry, forced};
     const size_t cs = b.csize, rs = cs / 2;
     const int64_t n = d.n, batch = d.batch;
     int st = GFFT_OK;
@@ -1078,6 +1131,220 @@ int enqueue_batch(gfft_batch_plan_s *p, const void *in, void *out, hipStream_t s
     return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, s);
 }
 
+
+// ---------------------------------------------------------------------------
+// Autotuner and tuning-table persistence
+// ---------------------------------------------------------------------------
+int tuning_load_locked(const char *path)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(GFFT_ERR_INVALID_ARG, "cannot open tuning file %s", path);
+    // one entry per line: {"precision": P, "n": N, "n1_log2": A, "col_m": B, "row_m": C, "pipe_mask": D, "row_variant": E, ...}
+    char line[1024];
+    int loaded = 0;
+    while (fgets(line, sizeof(line), f)) {
+        int prec, a, b, c, d, e;
+        long long n;
+        if (sscanf(line, " {\"precision\": %d, \"n\": %lld, \"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, "
+                         "\"pipe_mask\": %d, \"row_variant\": %d",
+                   &prec, &n, &a, &b, &c, &d, &e) == 7) {
+            PlanOpts o;
+            o.n1_log2 = a;
+            o.col_m = b;
+            o.row_m = c;
+            o.pipe_mask = d;
+            o.row_variant = e;
+            g_tune[{prec, (int64_t)n}] = o;
+            loaded++;
+        }
+    }
+    fclose(f);
+    return loaded >= 0 ? GFFT_OK : GFFT_ERR_INVALID_ARG;
+}
+
+// Candidate kernel choices for a C2C of length n.
+std::vector<PlanOpts> tune_candidates(int prec, int64_t n)
+{
+    std::vector<PlanOpts> out;
+    int cnt = 0;
+    const KernelEntry *tab = prec ? tile_kernels_f64(&cnt) : tile_kernels_f32(&cnt);
+    const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
+    if (n <= 1) return {PlanOpts{}};
+    if (n <= maxrow) {
+        for (int i = 0; i < cnt; i++)
+            if (tab[i].L == n && tab[i].inp == PAT_ROW && tab[i].outp == PAT_ROW) {
+                PlanOpts o;
+                o.row_variant = i;
+                out.push_back(o);
+            }
+        return out;
+    }
+    const int lg = ilog2(n), maxcol = max_len(prec, PAT_COL, PAT_COL), maxtr = max_len(prec, PAT_ROW, PAT_COL);
+    for (int l1 = 1; l1 < lg; l1++) {
+        const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << (lg - l1);
+        if (N1 > maxcol || N2 > maxtr) continue;
+        std::vector<int> cms{0}, rms{0};
+        bool cpipe = false, rpipe = false;
+        for (int i = 0; i < cnt; i++) {
+            const KernelEntry &e = tab[i];
+            if (e.L == N1 && e.inp == PAT_COL && e.outp == PAT_COL) {
+                if (std::find(cms.begin(), cms.end(), e.M) == cms.end()) cms.push_back(e.M);
+                cpipe = cpipe || e.persistent;
+            }
+            if (e.L == N2 && e.inp == PAT_ROW && e.outp == PAT_COL) {
+                if (std::find(rms.begin(), rms.end(), e.M) == rms.end()) rms.push_back(e.M);
+                rpipe = rpipe || e.persistent;
+            }
+        }
+        for (int cm : cms)
+            for (int rm : rms)
+                for (int pm = 0; pm < 4; pm++) {
+                    if (((pm & 1) && !cpipe) || ((pm & 2) && !rpipe)) continue;
+                    PlanOpts o;
+                    o.n1_log2 = l1;
+                    o.col_m = cm;
+                    o.row_m = rm;
+                    o.pipe_mask = pm;
+                    out.push_back(o);
+                }
+    }
+    return out;
+}
+
+}  // namespace
+}  // namespace gfft
+
+using namespace gfft;
+
+extern "C" {
+
+int gfft_autotune(int precision, int64_t n, int64_t batch, int reps, char *report, size_t len)
+{
+    if (precision != GFFT_F32 && precision != GFFT_F64) return fail(GFFT_ERR_INVALID_ARG, "bad precision");
+    if (!is_pow2(n) || n > ((int64_t)1 << 23) || batch < 1) return fail(GFFT_ERR_UNSUPPORTED, "bad size or batch");
+    if (reps < 1) reps = 10;
+    int dev = 0;
+    if (!device_ok(&dev)) return fail(GFFT_ERR_NO_DEVICE, "no HIP device available");
+    const size_t bytes = (size_t)n * batch * (precision ? 16 : 8);
+    void *in = nullptr, *out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipMalloc(&in, bytes));
+    HIP_TRY(hipMalloc(&out, bytes));
+    HIP_TRY(hipMemset(in, 0x3c, bytes));
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    std::string rep = "{\"precision\": " + std::to_string(precision) + ", \"n\": " + std::to_string(n) +
+                      ", \"batch\": " + std::to_string(batch) + ", \"candidates\": [\n";
+    float best_ms = 1e30f;
+    PlanOpts best;
+    int st = GFFT_OK, tried = 0;
+    for (const PlanOpts &o : tune_candidates(precision, n)) {
+        std::unique_ptr<gfft_plan_s> p(new gfft_plan_s());
+        p->d = gfft_desc{};
+        p->d.kind = GFFT_C2C;
+        p->d.precision = precision;
+        p->d.n = n;
+        p->d.batch = batch;
+        p->prec = precision;
+        p->device = dev;
+        if (hipDeviceGetAttribute(&p->num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            p->num_cus = 256;
+        if (build_plan(p.get(), false, &o) || validate(p.get())) continue;  // candidate not plannable
+        if (p->scratch_bytes && hipMalloc(&p->scratch, p->scratch_bytes) != hipSuccess) continue;
+        float ms = 0;
+        int r = run(p.get(), in, out, GFFT_FORWARD, s);  // warm-up
+        r = r ? r : run(p.get(), in, out, GFFT_FORWARD, s);
+        if (!r && hipEventRecord(e0, s) == hipSuccess) {
+            for (int i = 0; i < reps && !r; i++) r = run(p.get(), in, out, GFFT_FORWARD, s);
+            if (!r && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                ms /= reps;
+            else
+                r = GFFT_ERR_HIP;
+        }
+        if (p->scratch) (void)hipFree(p->scratch);
+        p->scratch = nullptr;
+        if (r) {
+            st = r;
+            break;
+        }
+        char line[256];
+        snprintf(line, sizeof(line),
+                 "  {\"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, \"pipe_mask\": %d, \"row_variant\": %d, "
+                 "\"ms\": %.5f}",
+                 o.n1_log2, o.col_m, o.row_m, o.pipe_mask, o.row_variant, ms);
+        rep += (tried ? ",\n" : "") + std::string(line);
+        tried++;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = o;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    (void)hipFree(in);
+    (void)hipFree(out);
+    if (st) return st;
+    if (!tried) return fail(GFFT_ERR_UNSUPPORTED, "no candidate could be planned");
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_loaded = true;
+        g_tune[{precision, n}] = best;
+    }
+    char tail[256];
+    snprintf(tail, sizeof(tail),
+             "\n], \"best\": {\"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, \"pipe_mask\": %d, \"row_variant\": %d, "
+             "\"ms\": %.5f}}",
+             best.n1_log2, best.col_m, best.row_m, best.pipe_mask, best.row_variant, best_ms);
+    rep += tail;
+    if (report && len) snprintf(report, len, "%s", rep.c_str());
+    return GFFT_OK;
+}
+
+int gfft_tuning_save(const char *path)
+{
+    if (!path) return fail(GFFT_ERR_INVALID_ARG, "null path");
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(GFFT_ERR_INVALID_ARG, "cannot write %s", path);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    fprintf(f, "{\"version\": 1, \"entries\": [\n");
+    size_t i = 0;
+    for (const auto &kv : g_tune) {
+        const PlanOpts &o = kv.second;
+        fprintf(f,
+                " {\"precision\": %d, \"n\": %lld, \"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, \"pipe_mask\": %d, "
+                "\"row_variant\": %d}%s\n",
+                kv.first.first, (long long)kv.first.second, o.n1_log2, o.col_m, o.row_m, o.pipe_mask, o.row_variant,
+                ++i < g_tune.size() ? "," : "");
+    }
+    fprintf(f, "]}\n");
+    fclose(f);
+    return GFFT_OK;
+}
+
+int gfft_tuning_load(const char *path)
+{
+    if (!path) return fail(GFFT_ERR_INVALID_ARG, "null path");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_loaded = true;
+    return tuning_load_locked(path);
+}
+
+int gfft_tuning_clear(void)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+    g_tune_loaded = true;
+    return GFFT_OK;
+}
+
+}  // extern "C"
+
+namespace gfft {
+namespace {
 }  // namespace
 }  // namespace gfft
 

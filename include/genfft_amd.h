@@ -168,6 +168,19 @@ GFFT_API int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out,
 GFFT_API int gfft_batch_plan_describe(gfft_batch_plan plan, char *buf, size_t len);
 GFFT_API int gfft_batch_plan_destroy(gfft_batch_plan plan);
 
+/* ---- autotuner (no reference counterpart) ---------------------------------
+ * Times every candidate kernel choice for a batched C2C of length n (four-step
+ * split, tile widths, pipelined kernels, radix order / threads of the
+ * single-pass kernel) on `batch` transforms, `reps` runs each, keeps the
+ * fastest in the process-wide tuning table that later gfft_plan_create calls
+ * consult, and writes a JSON report of all candidates to `report`.  The table
+ * is saved / loaded as JSON; GFFT_TUNING_FILE names a file loaded at the first
+ * plan creation. */
+GFFT_API int gfft_autotune(int precision, int64_t n, int64_t batch, int reps, char *report, size_t len);
+GFFT_API int gfft_tuning_save(const char *path);
+GFFT_API int gfft_tuning_load(const char *path);
+GFFT_API int gfft_tuning_clear(void);
+
 /* 1 if p is device (or managed) memory visible to the current device. */
 GFFT_API int gfft_is_device_pointer(const void *p);
 

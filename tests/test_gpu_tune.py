@@ -1,0 +1,49 @@
+"""The autotuner (gfft_autotune): candidate radix orders / tile widths /
+threads for single-pass lengths and four-step splits / pipelined kernels for
+long lengths are timed on the device; the tuned plan must keep parity with the
+oracle and the table must survive a JSON save / load round trip."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _parity(gpu, n, batch=3):
+    import genfft_amd as G
+    rng = np.random.default_rng(n)
+    x = (rng.uniform(-1, 1, (batch, n)) + 1j * rng.uniform(-1, 1, (batch, n))).astype(np.complex64)
+    for inv in (False, True):
+        y = G.fft(torch.from_numpy(x).to(gpu), inverse=inv).cpu().numpy()
+        assert O.rel_l2_rows(y, O.fft_ld(x.astype(np.complex128), inv)).max() <= O.tol_c64(n)
+
+
+@pytest.mark.parametrize("n,batch", [(1024, 4096), (1 << 16, 64)])
+def test_autotune_picks_fastest_and_keeps_parity(gpu, tmp_path, n, batch):
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rep = G.autotune(n, batch, torch.complex64, reps=5)
+    cands = rep["candidates"]
+    assert len(cands) >= 3
+    best = rep["best"]
+    assert best["ms"] == min(c["ms"] for c in cands)
+    _parity(gpu, n)
+    path = str(tmp_path / "tune.json")
+    G.save_tuning(path)
+    saved = json.load(open(path))
+    assert any(e["n"] == n for e in saved["entries"])
+    C.check(C.lib().gfft_tuning_clear())
+    G.load_tuning(path)
+    p = G.Plan(C.GFFT_C2C, n, batch=batch)
+    text = p.describe()
+    if n <= 16384:
+        assert "tile L=%d" % n in text
+    else:
+        assert f"L={1 << best['n1_log2']} " in text
+    _parity(gpu, n)
+    C.check(C.lib().gfft_tuning_clear())
