@@ -423,3 +423,8 @@ def fft2(x: torch.Tensor, inverse: bool = False) -> torch.Tensor:
     out = torch.empty_like(x)
     FFT2D(W, H, x.dtype).transform(out, W, x, W, inverse, batch=b)
     return out
+
+
+from .autograd import SpectralFilter, spectral_filter, spectral_filter_plain  # noqa: E402
+
+__all__ += ["SpectralFilter", "spectral_filter", "spectral_filter_plain"]

@@ -59,6 +59,10 @@ EXPORTS = (
     "gfft_tuning_save",
     "gfft_tuning_load",
     "gfft_tuning_clear",
+    "gfft_filter_plan_create",
+    "gfft_filter_forward",
+    "gfft_filter_backward",
+    "gfft_filter_plan_destroy",
 )
 
 GFFT_BATCH_GRAPH = 0
@@ -153,6 +157,14 @@ def lib() -> ctypes.CDLL:
         L.gfft_tuning_load.restype = i32
         L.gfft_tuning_clear.argtypes = []
         L.gfft_tuning_clear.restype = i32
+        L.gfft_filter_plan_create.argtypes = [ctypes.POINTER(vp), i32, i64, i64]
+        L.gfft_filter_plan_create.restype = i32
+        L.gfft_filter_forward.argtypes = [vp, vp, vp, vp, i32, vp]
+        L.gfft_filter_forward.restype = i32
+        L.gfft_filter_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.gfft_filter_backward.restype = i32
+        L.gfft_filter_plan_destroy.argtypes = [vp]
+        L.gfft_filter_plan_destroy.restype = i32
         _lib = L
         return L
 

@@ -1,0 +1,28 @@
+// gfft_filter.h -- fused spectral filter (kern_filter.hip).
+#pragma once
+#include <algorithm>
+
+#include "gfft_device.h"
+
+namespace gfft {
+
+struct FilterArgs {
+    const void *x;        // real rows (read as complex pairs), batch x n
+    const void *gy;       // adjoint: upstream gradient rows
+    void *y;              // forward: y rows; adjoint: gx rows; reduce: gH
+    const void *H;        // n/2 + 1 complex bins
+    void *partial;        // adjoint: per-row gH contributions, batch x (n/2 + 1)
+    const void *tw;       // W_n^k, k < n/2 (complex T, FP64-generated, rounded once)
+    const void *pass_tw;  // Stockham twiddles of the n/2-point FFT
+    int64_t batch;
+    int64_t bins;         // n/2 + 1
+    int32_t conj_h;       // forward kernel with conj(H) (the x-adjoint alone)
+    int32_t pad;
+};
+
+// which: 0 forward, 1 adjoint (gx + partials), 2 partial reduction (gH)
+hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s);
+// threads and radices of the fused kernel for L = n/2 (0 if unsupported)
+int filter_shape(int prec, int L, int *nt, int *rad);
+
+}  // namespace gfft

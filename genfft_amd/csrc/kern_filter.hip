@@ -1,0 +1,268 @@
+// kern_filter.hip -- fused spectral filter on real rows and its adjoint.
+//
+//   forward   y  = irfft(H * rfft(x))                         (length-n rows)
+//   adjoint   gx = irfft(conj(H) * rfft(gy))
+//             gH = sum_rows  w_k/n * rfft(gy)_k * conj(rfft(x)_k)   (w = 1 at k = 0, n/2; else 2)
+//
+// irfft follows the usual Hermitian convention: the imaginary parts of bins 0
+// and n/2 are ignored, and it is normalised (1/n), so y = IFFT(H_full * FFT(x))
+// / n with H_full the Hermitian extension of H.  gH is the gradient of a real
+// loss with respect to H in the (d/dRe + i d/dIm) convention.
+//
+// One workgroup owns one row and does everything in LDS: the n reals are read
+// as L = n/2 complex values, a forward L-point Stockham FFT gives Z, the
+// real-FFT unpack (the adjust_DIT_impl math, generic/fft_dit_impl_generic.inl:27-61)
+// gives the half spectrum, the filter is applied pairwise (bins k and L-k
+// belong to one thread), the c2r pack is applied in the same pass, and a
+// second L-point FFT (the inverse, by the re/im swap) produces the row.  One
+// HBM read and one write per row; the adjoint also reads x and writes a row
+// of gH partials that a small kernel sums over the rows.
+#include "gfft_filter.h"
+
+namespace gfft {
+namespace {
+
+template <typename T>
+__device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.re, -a.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.re * s, a.im * s}; }
+// a * i
+template <typename T>
+__device__ __forceinline__ cx<T> mul_i(cx<T> a) { return {-a.im, a.re}; }
+
+// real-FFT unpack of the pair (k, j = L - k), 0 < k < L/2:  F_k = E - i W^k O,
+// F_j = conj(E + i W^k O) with E = (Z_k + conj Z_j)/2, O = (Z_k - conj Z_j)/2
+template <typename T>
+__device__ __forceinline__ void unpack_pair(cx<T> zk, cx<T> zj, cx<T> w, cx<T> &fk, cx<T> &fj)
+{
+    const cx<T> E = cscale(zk + cconj(zj), (T)0.5);
+    const cx<T> O = cscale(zk - cconj(zj), (T)0.5);
+    const cx<T> t = mul_i(cmul(w, O));
+    fk = E - t;
+    fj = cconj(E + t);
+}
+
+// c2r pack of the pair: Z'_k = (Y_k + conj Y_j) + i W^-k (Y_k - conj Y_j),
+// Z'_j = (Y_j + conj Y_k) + i W^-j (Y_j - conj Y_k) with W^-j = -W^k.
+template <typename T>
+__device__ __forceinline__ void pack_pair(cx<T> yk, cx<T> yj, cx<T> w, cx<T> &zk, cx<T> &zj)
+{
+    zk = (yk + cconj(yj)) + mul_i(cmul(cconj(w), yk - cconj(yj)));
+    zj = (yj + cconj(yk)) - mul_i(cmul(w, yj - cconj(yk)));
+}
+
+template <typename T, int L, int NT, class G>
+__device__ __forceinline__ void load_row(cx<T> *lds, const cx<T> *row, bool swap, int tid)
+{
+    for (int n = tid; n < L; n += NT) {
+        const cx<T> v = row[n];
+        lds[G::off(0, n)] = swap ? swap_ri(v) : v;
+    }
+}
+
+// Spectrum of a row from its packed FFT Z in LDS: bins k and L-k of pair p.
+// p = 0 yields the real bins 0 and L (as .re of fk / fj).
+template <typename T, class G>
+__device__ __forceinline__ void row_bins(const cx<T> *lds, int p, int L, cx<T> w, cx<T> &fk, cx<T> &fj)
+{
+    if (p == 0) {
+        const cx<T> z0 = lds[G::off(0, 0)];
+        fk = {z0.re + z0.im, (T)0};
+        fj = {z0.re - z0.im, (T)0};
+        return;
+    }
+    unpack_pair(lds[G::off(0, p)], lds[G::off(0, L - p)], w, fk, fj);
+}
+
+template <typename T, int L, int NT, int... RS>
+__global__ void __launch_bounds__(NT) filter_fwd_kernel(FilterArgs a)
+{
+    using G = TileGeom<L, 1, LAY_SEQ, PAT_ROW, PAT_ROW>;
+    __shared__ cx<T> lds[G::ELEMS];
+    const int tid = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    const cx<T> *x = reinterpret_cast<const cx<T> *>(a.x) + row * L;
+    cx<T> *y = reinterpret_cast<cx<T> *>(a.y) + row * L;
+    const cx<T> *H = reinterpret_cast<const cx<T> *>(a.H);
+    const cx<T> *tw = reinterpret_cast<const cx<T> *>(a.tw);
+    const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
+    const TileArgs none{};
+    load_row<T, L, NT, G>(lds, x, false, tid);
+    __syncthreads();
+    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
+    // filter each pair of bins and pack for the inverse (stored swapped: the
+    // second forward FFT then computes the unscaled inverse)
+    for (int p = tid; p <= L / 2; p += NT) {
+        const cx<T> w = tw[p];
+        cx<T> fk, fj;
+        row_bins<T, G>(lds, p, L, w, fk, fj);
+        if (p == 0) {
+            const T y0 = H[0].re * fk.re, yl = H[L].re * fj.re;  // imaginary parts of bins 0, L dropped
+            lds[G::off(0, 0)] = swap_ri(cx<T>{y0 + yl, y0 - yl});
+        } else {
+            const cx<T> hk = H[p], hj = H[L - p];
+            const cx<T> yk = cmul(a.conj_h ? cconj(hk) : hk, fk);
+            const cx<T> yj = cmul(a.conj_h ? cconj(hj) : hj, fj);
+            cx<T> zk, zj;
+            pack_pair(yk, yj, w, zk, zj);
+            lds[G::off(0, p)] = swap_ri(zk);
+            if (L - p != p) lds[G::off(0, L - p)] = swap_ri(zj);
+        }
+    }
+    __syncthreads();
+    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
+    const T s = (T)1 / (T)(2 * L);
+    for (int n = tid; n < L; n += NT) y[n] = cscale(swap_ri(lds[G::off(0, n)]), s);
+}
+
+template <typename T, int L, int NT, int... RS>
+__global__ void __launch_bounds__(NT) filter_bwd_kernel(FilterArgs a)
+{
+    using G = TileGeom<L, 1, LAY_SEQ, PAT_ROW, PAT_ROW>;
+    __shared__ cx<T> gl[G::ELEMS];  // rfft(gy), then the gx pass
+    __shared__ cx<T> xl[G::ELEMS];  // rfft(x)
+    const int tid = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    const cx<T> *gy = reinterpret_cast<const cx<T> *>(a.gy) + row * L;
+    const cx<T> *x = reinterpret_cast<const cx<T> *>(a.x) + row * L;
+    cx<T> *gx = reinterpret_cast<cx<T> *>(a.y) + row * L;
+    cx<T> *part = reinterpret_cast<cx<T> *>(a.partial) + row * (L + 1);
+    const cx<T> *H = reinterpret_cast<const cx<T> *>(a.H);
+    const cx<T> *tw = reinterpret_cast<const cx<T> *>(a.tw);
+    const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
+    const TileArgs none{};
+    load_row<T, L, NT, G>(gl, gy, false, tid);
+    load_row<T, L, NT, G>(xl, x, false, tid);
+    __syncthreads();
+    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(gl, ptw, tid, none, 0, 0, Tw4Pre{});
+    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(xl, ptw, tid, none, 0, 0, Tw4Pre{});
+    const T inv_n = (T)1 / (T)(2 * L);
+    for (int p = tid; p <= L / 2; p += NT) {
+        const cx<T> w = tw[p];
+        cx<T> gk, gj, xk, xj;
+        row_bins<T, G>(gl, p, L, w, gk, gj);
+        row_bins<T, G>(xl, p, L, w, xk, xj);
+        if (p == 0) {
+            // bins 0 and L are real in both spectra; weight 1
+            part[0] = {gk.re * xk.re * inv_n, (T)0};
+            part[L] = {gj.re * xj.re * inv_n, (T)0};
+            const T y0 = H[0].re * gk.re, yl = H[L].re * gj.re;
+            gl[G::off(0, 0)] = swap_ri(cx<T>{y0 + yl, y0 - yl});
+        } else {
+            part[p] = cscale(cmul(gk, cconj(xk)), 2 * inv_n);
+            if (L - p != p) part[L - p] = cscale(cmul(gj, cconj(xj)), 2 * inv_n);
+            const cx<T> yk = cmul(cconj(H[p]), gk), yj = cmul(cconj(H[L - p]), gj);
+            cx<T> zk, zj;
+            pack_pair(yk, yj, w, zk, zj);
+            gl[G::off(0, p)] = swap_ri(zk);
+            if (L - p != p) gl[G::off(0, L - p)] = swap_ri(zj);
+        }
+    }
+    __syncthreads();
+    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(gl, ptw, tid, none, 0, 0, Tw4Pre{});
+    for (int n = tid; n < L; n += NT) gx[n] = cscale(swap_ri(gl[G::off(0, n)]), inv_n);
+}
+
+// gH[k] = sum over rows of the partials (fixed order: deterministic)
+template <typename T>
+__global__ void __launch_bounds__(256) filter_reduce_kernel(FilterArgs a)
+{
+    const int64_t bins = a.bins;
+    const cx<T> *part = reinterpret_cast<const cx<T> *>(a.partial);
+    cx<T> *gH = reinterpret_cast<cx<T> *>(a.y);
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < bins; k += (int64_t)gridDim.x * blockDim.x) {
+        cx<T> s = {(T)0, (T)0};
+        for (int64_t b = 0; b < a.batch; b++) s = s + part[b * bins + k];
+        gH[k] = s;
+    }
+}
+
+template <typename T>
+struct FilterTable {
+    int L, NT, nrad, rad[4];
+    void (*fwd)(FilterArgs);
+    void (*bwd)(FilterArgs);
+};
+
+template <typename T, int L, int NT, int... RS>
+constexpr FilterTable<T> ft()
+{
+    FilterTable<T> e{};
+    e.L = L;
+    e.NT = NT;
+    e.nrad = sizeof...(RS);
+    const int r[] = {RS...};
+    for (int i = 0; i < (int)sizeof...(RS); i++) e.rad[i] = r[i];
+    e.fwd = &filter_fwd_kernel<T, L, NT, RS...>;
+    e.bwd = &filter_bwd_kernel<T, L, NT, RS...>;
+    return e;
+}
+
+const FilterTable<float> kF32[] = {
+    ft<float, 32, 64, 8, 4>(),       ft<float, 64, 64, 8, 8>(),        ft<float, 128, 64, 16, 8>(),
+    ft<float, 256, 64, 16, 16>(),    ft<float, 512, 64, 8, 8, 8>(),    ft<float, 1024, 64, 16, 16, 4>(),
+    ft<float, 2048, 128, 16, 16, 8>(), ft<float, 4096, 256, 16, 16, 16>(), ft<float, 8192, 512, 16, 16, 8, 4>(),
+};
+const FilterTable<double> kF64[] = {
+    ft<double, 32, 64, 8, 4>(),      ft<double, 64, 64, 8, 8>(),       ft<double, 128, 64, 16, 8>(),
+    ft<double, 256, 64, 16, 16>(),   ft<double, 512, 64, 8, 8, 8>(),   ft<double, 1024, 64, 16, 16, 4>(),
+    ft<double, 2048, 128, 16, 16, 8>(), ft<double, 4096, 256, 16, 16, 16>(),
+};
+
+template <typename T, size_t K>
+const FilterTable<T> *lookup(const FilterTable<T> (&tab)[K], int L)
+{
+    for (size_t i = 0; i < K; i++)
+        if (tab[i].L == L) return &tab[i];
+    return nullptr;
+}
+
+}  // namespace
+
+int filter_shape(int prec, int L, int *nt, int *rad)
+{
+    int nrad = 0;
+    if (prec) {
+        const auto *e = lookup(kF64, L);
+        if (!e) return 0;
+        *nt = e->NT;
+        nrad = e->nrad;
+        for (int i = 0; i < nrad; i++) rad[i] = e->rad[i];
+    } else {
+        const auto *e = lookup(kF32, L);
+        if (!e) return 0;
+        *nt = e->NT;
+        nrad = e->nrad;
+        for (int i = 0; i < nrad; i++) rad[i] = e->rad[i];
+    }
+    return nrad;
+}
+
+hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s)
+{
+    const int L = (int)(a.bins - 1);
+    void (*fn)(FilterArgs) = nullptr;
+    int nt = 0;
+    if (which == 2) {
+        const unsigned g = (unsigned)std::min<int64_t>(1024, (a.bins + 255) / 256);
+        if (prec) hipLaunchKernelGGL(filter_reduce_kernel<double>, dim3(g), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(filter_reduce_kernel<float>, dim3(g), dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
+    if (prec) {
+        const auto *e = lookup(kF64, L);
+        if (!e) return hipErrorInvalidValue;
+        fn = which ? e->bwd : e->fwd;
+        nt = e->NT;
+    } else {
+        const auto *e = lookup(kF32, L);
+        if (!e) return hipErrorInvalidValue;
+        fn = which ? e->bwd : e->fwd;
+        nt = e->NT;
+    }
+    FilterArgs args = a;
+    void *kargs[] = {&args};
+    return hipLaunchKernel((const void *)fn, dim3((unsigned)a.batch), dim3(nt), kargs, 0, s);
+}
+
+}  // namespace gfft

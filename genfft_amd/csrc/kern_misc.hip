@@ -78,7 +78,8 @@ __global__ void __launch_bounds__(256) unpack_kernel(UnpackArgs a)
 }
 
 // Z[k] = (F[k] + conj F[L-k]) + i W_N^-k (F[k] - conj F[L-k]),  L = N/2.
-// The unscaled inverse L-point FFT of Z, read as N reals, is N * x.
+// The unscaled inverse L-point FFT of Z, read as N reals, is N * x.  As in
+// any c2r, the imaginary parts of bins 0 and L are ignored.
 template <typename T>
 __global__ void __launch_bounds__(256) pack_kernel(UnpackArgs a)
 {
@@ -91,7 +92,11 @@ __global__ void __launch_bounds__(256) pack_kernel(UnpackArgs a)
          g += (int64_t)gridDim.x * blockDim.x) {
         const int64_t b = g / L, k = g - b * L;
         const C *f = F + b * a.in_dist;
-        const C fk = f[k], fm = f[L - k];
+        C fk = f[k], fm = f[L - k];
+        if (k == 0) {  // bins 0 and L of a Hermitian spectrum are real: ignore their imaginary parts
+            fk.im = (T)0;
+            fm.im = (T)0;
+        }
         const C s = fk + conj(fm);
         const C d = fk - conj(fm);
         const cx<double> w = twl(a.tw, (uint64_t)k);

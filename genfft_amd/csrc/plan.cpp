@@ -39,6 +39,7 @@
 
 #include "../../include/genfft_amd.h"
 #include "gfft_batch.h"
+#include "gfft_filter.h"
 #include "gfft_misc.h"
 #include "gfft_registry.h"
 
@@ -1655,6 +1656,107 @@ int gfft_batch_plan_describe(gfft_batch_plan plan, char *buf, size_t len)
     snprintf(head, sizeof(head), "batch jobs=%lld scratch_bytes=%zu offsets_even=%d\n", (long long)plan->count,
              plan->scratch_bytes, (int)plan->offsets_even);
     return describe_launches(plan->launches, head, buf, len);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Fused spectral filter (kern_filter.hip)
+// ===========================================================================
+struct gfft_filter_plan_s {
+    int prec = 0, device = 0;
+    int64_t n = 0, batch = 0;
+    void *tw = nullptr;        // W_n^k, k < n/2
+    const void *pass_tw = nullptr;
+    void *partial = nullptr;   // batch x (n/2 + 1) complex
+};
+
+extern "C" {
+
+int gfft_filter_plan_create(gfft_filter_plan *plan, int precision, int64_t n, int64_t batch)
+{
+    if (!plan) return fail(GFFT_ERR_INVALID_ARG, "null plan");
+    *plan = nullptr;
+    if (precision != GFFT_F32 && precision != GFFT_F64) return fail(GFFT_ERR_INVALID_ARG, "bad precision");
+    if (batch < 1 || batch > INT32_MAX) return fail(GFFT_ERR_INVALID_ARG, "bad batch");
+    const int64_t L = n / 2;
+    KernelEntry k{};
+    k.prec = precision;
+    k.L = (int)L;
+    int nt = 0;
+    if (!is_pow2(n) || n < 64 || (k.nrad = filter_shape(precision, (int)L, &nt, k.rad)) == 0)
+        return fail(GFFT_ERR_UNSUPPORTED, "fused filter length %lld not supported (64 .. %d)", (long long)n,
+                    precision ? 8192 : 16384);
+    int dev = 0;
+    if (!device_ok(&dev)) return fail(GFFT_ERR_NO_DEVICE, "no HIP device available");
+    std::unique_ptr<gfft_filter_plan_s> p(new gfft_filter_plan_s());
+    p->prec = precision;
+    p->device = dev;
+    p->n = n;
+    p->batch = batch;
+    int st = get_pass_twiddles(dev, k, &p->pass_tw, false);
+    if (st) return st;
+    std::vector<long double> re(L), im(L);
+    for (int64_t m = 0; m < L; m++) twiddle_ld(m, n, &re[m], &im[m]);
+    if (precision) {
+        std::vector<double> h(2 * L);
+        for (int64_t m = 0; m < L; m++) h[2 * m] = (double)re[m], h[2 * m + 1] = (double)im[m];
+        st = upload(&p->tw, h.data(), h.size() * sizeof(double));
+    } else {
+        std::vector<float> h(2 * L);
+        for (int64_t m = 0; m < L; m++) h[2 * m] = (float)re[m], h[2 * m + 1] = (float)im[m];
+        st = upload(&p->tw, h.data(), h.size() * sizeof(float));
+    }
+    if (st) return st;
+    const size_t cs = precision ? 16 : 8;
+    HIP_TRY(hipMalloc(&p->partial, (size_t)batch * (L + 1) * cs));
+    *plan = p.release();
+    return GFFT_OK;
+}
+
+int gfft_filter_plan_destroy(gfft_filter_plan plan)
+{
+    if (!plan) return GFFT_OK;
+    if (plan->tw) (void)hipFree(plan->tw);
+    if (plan->partial) (void)hipFree(plan->partial);
+    delete plan;
+    return GFFT_OK;
+}
+
+int gfft_filter_forward(gfft_filter_plan plan, const void *x, const void *H, void *y, int conj_h, void *stream)
+{
+    if (!plan || !x || !H || !y) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    FilterArgs a{};
+    a.x = x;
+    a.H = H;
+    a.y = y;
+    a.tw = plan->tw;
+    a.pass_tw = plan->pass_tw;
+    a.batch = plan->batch;
+    a.bins = plan->n / 2 + 1;
+    a.conj_h = conj_h ? 1 : 0;
+    HIP_TRY(launch_filter(plan->prec, 0, a, (hipStream_t)stream));
+    return GFFT_OK;
+}
+
+int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, const void *gy, void *gx, void *gH,
+                         void *stream)
+{
+    if (!plan || !x || !H || !gy || !gx || !gH) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    FilterArgs a{};
+    a.x = x;
+    a.gy = gy;
+    a.H = H;
+    a.y = gx;
+    a.partial = plan->partial;
+    a.tw = plan->tw;
+    a.pass_tw = plan->pass_tw;
+    a.batch = plan->batch;
+    a.bins = plan->n / 2 + 1;
+    HIP_TRY(launch_filter(plan->prec, 1, a, (hipStream_t)stream));
+    a.y = gH;
+    HIP_TRY(launch_filter(plan->prec, 2, a, (hipStream_t)stream));
+    return GFFT_OK;
 }
 
 }  // extern "C"

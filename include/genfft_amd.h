@@ -168,6 +168,24 @@ GFFT_API int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out,
 GFFT_API int gfft_batch_plan_describe(gfft_batch_plan plan, char *buf, size_t len);
 GFFT_API int gfft_batch_plan_destroy(gfft_batch_plan plan);
 
+/* ---- fused spectral filter and its adjoint (no reference counterpart) ------
+ * On `batch` real rows of length n (64 .. 16384 complex64 path / 8192
+ * complex128 path), H holding the n/2+1 bins of a Hermitian filter:
+ *   forward   y  = irfft(H * rfft(x))      (conj(H) when conj_h != 0)
+ *   backward  gx = irfft(conj(H) * rfft(gy)),
+ *             gH = sum_rows w_k/n * rfft(gy)_k * conj(rfft(x)_k), w = 1 at k = 0, n/2, else 2
+ * irfft is normalised (1/n) and ignores the imaginary parts of bins 0 and n/2.
+ * Each row is one workgroup doing FFT -> filter -> inverse FFT in LDS: one
+ * HBM read and one write per row.  gH is the gradient of a real loss with
+ * respect to H (d/dRe + i d/dIm).  Rows are contiguous (distance n reals). */
+typedef struct gfft_filter_plan_s *gfft_filter_plan;
+GFFT_API int gfft_filter_plan_create(gfft_filter_plan *plan, int precision, int64_t n, int64_t batch);
+GFFT_API int gfft_filter_forward(gfft_filter_plan plan, const void *x, const void *H, void *y, int conj_h,
+                                 void *stream);
+GFFT_API int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, const void *gy, void *gx,
+                                  void *gH, void *stream);
+GFFT_API int gfft_filter_plan_destroy(gfft_filter_plan plan);
+
 /* ---- autotuner (no reference counterpart) ---------------------------------
  * Times every candidate kernel choice for a batched C2C of length n (four-step
  * split, tile widths, pipelined kernels, radix order / threads of the

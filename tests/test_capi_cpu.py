@@ -157,3 +157,20 @@ def test_bench_shape_structure():
     t = tiles(text)
     assert field(t[0], "L") == "1024" and field(t[1], "L") == "1024"
     assert "scratch_bytes=" in text
+
+
+def test_filter_plan_argument_checks():
+    """Argument and size checks of the fused filter come before any device use."""
+    import torch
+    lib = C.lib()
+    h = ctypes.c_void_p()
+    assert lib.gfft_filter_plan_create(None, C.GFFT_F32, 1024, 1) == C.GFFT_ERR_INVALID_ARG
+    assert lib.gfft_filter_plan_create(ctypes.byref(h), 7, 1024, 1) == C.GFFT_ERR_INVALID_ARG
+    assert lib.gfft_filter_plan_create(ctypes.byref(h), C.GFFT_F32, 1024, 0) == C.GFFT_ERR_INVALID_ARG
+    for prec, n in [(C.GFFT_F32, 100), (C.GFFT_F32, 32), (C.GFFT_F32, 32768), (C.GFFT_F64, 16384)]:
+        assert lib.gfft_filter_plan_create(ctypes.byref(h), prec, n, 4) == C.GFFT_ERR_UNSUPPORTED
+        assert b"fused filter" in lib.gfft_last_error()
+    assert lib.gfft_filter_forward(None, None, None, None, 0, None) == C.GFFT_ERR_INVALID_ARG
+    assert lib.gfft_filter_backward(None, None, None, None, None, None, None) == C.GFFT_ERR_INVALID_ARG
+    if not torch.cuda.is_available():
+        assert lib.gfft_filter_plan_create(ctypes.byref(h), C.GFFT_F32, 1024, 4) == C.GFFT_ERR_NO_DEVICE

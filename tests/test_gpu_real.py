@@ -161,3 +161,24 @@ def test_real_full_size_properties(gpu):
     for r in (0, 255):
         ref = O.fft_ld(x[r].cpu().numpy().astype(np.complex128))[: n // 2 + 1]
         assert O.rel_l2(h[r].cpu().numpy(), ref) <= O.tol_c64(n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [4, 64, 4096])
+def test_c2r_ignores_imag_of_dc_and_nyquist(gpu, prec, n):
+    """A c2r reads bins 0 and n/2 as real (the Hermitian convention): random
+    imaginary parts there leave the output unchanged, and the output equals
+    the inverse DFT of the Hermitian extension built from the real parts."""
+    import genfft_amd as G
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    rng = np.random.default_rng(n)
+    F = (rng.standard_normal((2, n // 2 + 1)) + 1j * rng.standard_normal((2, n // 2 + 1))).astype(cdt)
+    Fr = F.copy()
+    Fr[:, 0] = F[:, 0].real
+    Fr[:, n // 2] = F[:, n // 2].real
+    a = G.irfft(torch.from_numpy(F).to(gpu), n).cpu().numpy()
+    b = G.irfft(torch.from_numpy(Fr).to(gpu), n).cpu().numpy()
+    assert np.array_equal(a, b)
+    full = np.concatenate([Fr, np.conj(Fr[:, 1:n // 2][:, ::-1])], axis=1).astype(np.complex128)
+    ref = O.fft_ld(full, inv=True).real
+    assert O.rel_l2(a, ref) <= _tol(prec, n)
