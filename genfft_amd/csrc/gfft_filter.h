@@ -17,7 +17,13 @@ struct FilterArgs {
     int64_t batch;
     int64_t bins;         // n/2 + 1
     int32_t conj_h;       // forward kernel with conj(H) (the x-adjoint alone)
-    int32_t pad;
+    int32_t vec;          // all row pointers 16-byte aligned: vector loads/stores
+};
+
+struct ReduceArgs {
+    const void *src;      // rows x pitch complex partials
+    void *dst;            // one sum row per segment, out_pitch apart
+    int64_t bins, rows, pitch, seg, out_pitch;
 };
 
 // which: 0 forward, 1 adjoint (gx + partials), 2 partial reduction (gH)

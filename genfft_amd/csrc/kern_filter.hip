@@ -51,12 +51,40 @@ __device__ __forceinline__ void pack_pair(cx<T> yk, cx<T> yj, cx<T> w, cx<T> &zk
     zj = (yj + cconj(yk)) - mul_i(cmul(w, yj - cconj(yk)));
 }
 
+// Row <-> LDS.  complex64 rows move as 16-byte vectors (two elements per
+// lane) when every row pointer is 16-byte aligned.
 template <typename T, int L, int NT, class G>
-__device__ __forceinline__ void load_row(cx<T> *lds, const cx<T> *row, bool swap, int tid)
+__device__ __forceinline__ void load_row(cx<T> *lds, const cx<T> *row, bool vec, int tid)
 {
-    for (int n = tid; n < L; n += NT) {
-        const cx<T> v = row[n];
-        lds[G::off(0, n)] = swap ? swap_ri(v) : v;
+    if (sizeof(T) == 4 && vec) {
+        const float4 *r4 = reinterpret_cast<const float4 *>(row);
+#pragma unroll
+        for (int q = tid; q < L / 2; q += NT) {
+            const float4 v = r4[q];
+            lds[G::off(0, 2 * q)] = cx<T>{(T)v.x, (T)v.y};
+            lds[G::off(0, 2 * q + 1)] = cx<T>{(T)v.z, (T)v.w};
+        }
+    } else {
+#pragma unroll
+        for (int n = tid; n < L; n += NT) lds[G::off(0, n)] = row[n];
+    }
+}
+
+// row[n] = s * swap(lds[n]): the second FFT ran on swapped data (inverse)
+template <typename T, int L, int NT, class G>
+__device__ __forceinline__ void store_row(cx<T> *row, const cx<T> *lds, T s, bool vec, int tid)
+{
+    if (sizeof(T) == 4 && vec) {
+        float4 *r4 = reinterpret_cast<float4 *>(row);
+#pragma unroll
+        for (int q = tid; q < L / 2; q += NT) {
+            const cx<T> a = cscale(swap_ri(lds[G::off(0, 2 * q)]), s);
+            const cx<T> b = cscale(swap_ri(lds[G::off(0, 2 * q + 1)]), s);
+            r4[q] = float4{(float)a.re, (float)a.im, (float)b.re, (float)b.im};
+        }
+    } else {
+#pragma unroll
+        for (int n = tid; n < L; n += NT) row[n] = cscale(swap_ri(lds[G::off(0, n)]), s);
     }
 }
 
@@ -87,7 +115,7 @@ __global__ void __launch_bounds__(NT) filter_fwd_kernel(FilterArgs a)
     const cx<T> *tw = reinterpret_cast<const cx<T> *>(a.tw);
     const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
     const TileArgs none{};
-    load_row<T, L, NT, G>(lds, x, false, tid);
+    load_row<T, L, NT, G>(lds, x, a.vec, tid);
     __syncthreads();
     stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
     // filter each pair of bins and pack for the inverse (stored swapped: the
@@ -111,8 +139,7 @@ __global__ void __launch_bounds__(NT) filter_fwd_kernel(FilterArgs a)
     }
     __syncthreads();
     stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
-    const T s = (T)1 / (T)(2 * L);
-    for (int n = tid; n < L; n += NT) y[n] = cscale(swap_ri(lds[G::off(0, n)]), s);
+    store_row<T, L, NT, G>(y, lds, (T)1 / (T)(2 * L), a.vec, tid);
 }
 
 template <typename T, int L, int NT, int... RS>
@@ -131,8 +158,8 @@ __global__ void __launch_bounds__(NT) filter_bwd_kernel(FilterArgs a)
     const cx<T> *tw = reinterpret_cast<const cx<T> *>(a.tw);
     const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
     const TileArgs none{};
-    load_row<T, L, NT, G>(gl, gy, false, tid);
-    load_row<T, L, NT, G>(xl, x, false, tid);
+    load_row<T, L, NT, G>(gl, gy, a.vec, tid);
+    load_row<T, L, NT, G>(xl, x, a.vec, tid);
     __syncthreads();
     stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(gl, ptw, tid, none, 0, 0, Tw4Pre{});
     stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(xl, ptw, tid, none, 0, 0, Tw4Pre{});
@@ -160,20 +187,35 @@ __global__ void __launch_bounds__(NT) filter_bwd_kernel(FilterArgs a)
     }
     __syncthreads();
     stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(gl, ptw, tid, none, 0, 0, Tw4Pre{});
-    for (int n = tid; n < L; n += NT) gx[n] = cscale(swap_ri(gl[G::off(0, n)]), inv_n);
+    store_row<T, L, NT, G>(gx, gl, inv_n, a.vec, tid);
 }
 
-// gH[k] = sum over rows of the partials (fixed order: deterministic)
+// dst[y * out_pitch + k] = sum of src[b * pitch + k] over the rows b of
+// segment y (blockIdx.y).  A block covers 32 bins with 8 row lanes; the lanes
+// are combined in a fixed order, so the sum is deterministic.  The first pass
+// may write in place into the first row of its own segment: only this block
+// reads that region, and all of its reads precede the barrier.
 template <typename T>
-__global__ void __launch_bounds__(256) filter_reduce_kernel(FilterArgs a)
+__global__ void __launch_bounds__(256) filter_reduce_kernel(ReduceArgs r)
 {
-    const int64_t bins = a.bins;
-    const cx<T> *part = reinterpret_cast<const cx<T> *>(a.partial);
-    cx<T> *gH = reinterpret_cast<cx<T> *>(a.y);
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < bins; k += (int64_t)gridDim.x * blockDim.x) {
-        cx<T> s = {(T)0, (T)0};
-        for (int64_t b = 0; b < a.batch; b++) s = s + part[b * bins + k];
-        gH[k] = s;
+    __shared__ cx<T> acc[8][32];
+    const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    const int64_t k = blockIdx.x * 32ll + lane;
+    const int64_t b0 = (int64_t)blockIdx.y * r.seg;
+    const int64_t b1 = b0 + r.seg < r.rows ? b0 + r.seg : r.rows;
+    const cx<T> *src = reinterpret_cast<const cx<T> *>(r.src);
+    cx<T> s = {(T)0, (T)0};
+    if (k < r.bins) {
+#pragma unroll 8
+        for (int64_t b = b0 + grp; b < b1; b += 8) s = s + src[b * r.pitch + k];
+    }
+    acc[grp][lane] = s;
+    __syncthreads();
+    if (grp == 0 && k < r.bins) {
+        cx<T> t = acc[0][lane];
+#pragma unroll
+        for (int g = 1; g < 8; g++) t = t + acc[g][lane];
+        reinterpret_cast<cx<T> *>(r.dst)[(int64_t)blockIdx.y * r.out_pitch + k] = t;
     }
 }
 
@@ -243,10 +285,32 @@ hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s
     const int L = (int)(a.bins - 1);
     void (*fn)(FilterArgs) = nullptr;
     int nt = 0;
-    if (which == 2) {
-        const unsigned g = (unsigned)std::min<int64_t>(1024, (a.bins + 255) / 256);
-        if (prec) hipLaunchKernelGGL(filter_reduce_kernel<double>, dim3(g), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(filter_reduce_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    if (which == 2) {  // gH from the per-row partials: one or two reduction passes
+        const int64_t bins = a.bins, rows = a.batch;
+        const int64_t gx = (bins + 31) / 32;
+        int64_t S = std::min<int64_t>((1024 + gx - 1) / gx, (rows + 63) / 64);
+        if (S < 1) S = 1;
+        const int64_t seg = (rows + S - 1) / S;
+        S = (rows + seg - 1) / seg;
+        ReduceArgs r{};
+        r.src = a.partial;
+        r.bins = bins;
+        r.rows = rows;
+        r.pitch = bins;
+        r.seg = seg;
+        if (S > 1) {
+            r.dst = a.partial;  // segment sums into the first row of each segment
+            r.out_pitch = seg * bins;
+            if (prec) hipLaunchKernelGGL(filter_reduce_kernel<double>, dim3((unsigned)gx, (unsigned)S), dim3(256), 0, s, r);
+            else hipLaunchKernelGGL(filter_reduce_kernel<float>, dim3((unsigned)gx, (unsigned)S), dim3(256), 0, s, r);
+            r.rows = S;
+            r.pitch = seg * bins;
+            r.seg = S;
+        }
+        r.dst = a.y;
+        r.out_pitch = 0;
+        if (prec) hipLaunchKernelGGL(filter_reduce_kernel<double>, dim3((unsigned)gx, 1), dim3(256), 0, s, r);
+        else hipLaunchKernelGGL(filter_reduce_kernel<float>, dim3((unsigned)gx, 1), dim3(256), 0, s, r);
         return hipGetLastError();
     }
     if (prec) {

@@ -1735,6 +1735,7 @@ int gfft_filter_forward(gfft_filter_plan plan, const void *x, const void *H, voi
     a.batch = plan->batch;
     a.bins = plan->n / 2 + 1;
     a.conj_h = conj_h ? 1 : 0;
+    a.vec = ((((uintptr_t)x) | ((uintptr_t)y)) & 15) == 0;
     HIP_TRY(launch_filter(plan->prec, 0, a, (hipStream_t)stream));
     return GFFT_OK;
 }
@@ -1753,6 +1754,7 @@ int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, co
     a.pass_tw = plan->pass_tw;
     a.batch = plan->batch;
     a.bins = plan->n / 2 + 1;
+    a.vec = ((((uintptr_t)x) | ((uintptr_t)gy) | ((uintptr_t)gx)) & 15) == 0;
     HIP_TRY(launch_filter(plan->prec, 1, a, (hipStream_t)stream));
     a.y = gH;
     HIP_TRY(launch_filter(plan->prec, 2, a, (hipStream_t)stream));
