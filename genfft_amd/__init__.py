@@ -84,6 +84,21 @@ class Plan:
                                            C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
                                            s.cuda_stream))
 
+    def execute_scaled(self, inp: torch.Tensor, out: torch.Tensor, scales: torch.Tensor, auto: bool,
+                       inverse: bool = False, stream=None) -> None:
+        """16-bit storage plans: out_b = scales[b] * FFT(in_b); with ``auto`` the
+        library first writes power-of-two scales that keep FP16 finite."""
+        dev = _check_gpu(inp, out, scales)
+        if scales.dtype != torch.float32 or not scales.is_contiguous() or scales.numel() < self.desc.batch:
+            raise ValueError("scales must be a contiguous float32 tensor of batch elements")
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        with torch.cuda.device(dev):
+            C.check(self._lib.gfft_execute_scaled(self._h, inp.data_ptr(), out.data_ptr(),
+                                                  C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
+                                                  scales.data_ptr(),
+                                                  C.GFFT_SCALE_AUTO if auto else C.GFFT_SCALE_GIVEN,
+                                                  s.cuda_stream))
+
     def execute_ptr(self, in_ptr: int, out_ptr: int, inverse: bool = False, stream_handle: int = 0) -> None:
         C.check(self._lib.gfft_execute(self._h, in_ptr, out_ptr,
                                        C.GFFT_INVERSE if inverse else C.GFFT_FORWARD, stream_handle))
@@ -428,3 +443,62 @@ def fft2(x: torch.Tensor, inverse: bool = False) -> torch.Tensor:
 from .autograd import SpectralFilter, spectral_filter, spectral_filter_plain  # noqa: E402
 
 __all__ += ["SpectralFilter", "spectral_filter", "spectral_filter_plain"]
+
+
+# ---------------------------------------------------------------------------
+# 16-bit storage (FP16 / BF16 pairs, FP32 arithmetic); no reference counterpart
+# ---------------------------------------------------------------------------
+_HALF = {torch.float16: C.GFFT_F16, torch.bfloat16: C.GFFT_BF16}
+
+
+def _as_pairs(x: torch.Tensor) -> torch.Tensor:
+    if x.dtype == torch.complex32:
+        return torch.view_as_real(x)
+    if x.dtype not in _HALF or x.shape[-1] != 2:
+        raise TypeError("16-bit transforms take complex32 or float16/bfloat16 [..., n, 2] tensors")
+    return x
+
+
+def _run16(kind, x, n, n2, batch, inverse, scale):
+    pairs = _as_pairs(x).contiguous()
+    dev = _check_gpu(pairs)
+    prec = _HALF[pairs.dtype]
+    plan = _plan(kind, n, prec, dev, n2=n2, batch=batch)
+    out = torch.empty_like(pairs)
+    if scale is None:
+        plan.execute(pairs, out, inverse)
+        scales = None
+    else:
+        auto = isinstance(scale, str)
+        if auto and scale != "auto":
+            raise ValueError("scale must be None, 'auto' or a float32 tensor of batch factors")
+        scales = torch.empty(batch, dtype=torch.float32, device=dev) if auto else \
+            scale.to(device=dev, dtype=torch.float32).contiguous()
+        plan.execute_scaled(pairs, out, scales, auto, inverse)
+    if x.dtype == torch.complex32:
+        out = out.view(torch.complex32).squeeze(-1)
+    return out, scales
+
+
+def fft16(x: torch.Tensor, inverse: bool = False, scale=None):
+    """Batched 1D FFT over the last complex axis with 16-bit storage.
+
+    ``x``: complex32 [..., n] or float16 / bfloat16 [..., n, 2].  ``scale``:
+    None (no scaling), "auto" (per-transform power of two keeping FP16
+    finite) or a float32 tensor with one factor per transform.  Returns
+    (y, scales) with y = scales[b] * FFT(x_b) in x's layout."""
+    n = x.shape[-1] if x.dtype == torch.complex32 else x.shape[-2]
+    batch = max(1, (x.numel() if x.dtype == torch.complex32 else x.numel() // 2) // n)
+    return _run16(C.GFFT_C2C, x, n, 0, batch, inverse, scale)
+
+
+def fft2_16(x: torch.Tensor, inverse: bool = False, scale=None):
+    """2D FFT over the last two complex axes (rows, cols) with 16-bit storage;
+    one scale per image.  Same layouts and scale modes as fft16."""
+    cplx = x.dtype == torch.complex32
+    H, W = (x.shape[-2], x.shape[-1]) if cplx else (x.shape[-3], x.shape[-2])
+    batch = max(1, (x.numel() if cplx else x.numel() // 2) // (H * W))
+    return _run16(C.GFFT_C2C_2D, x, W, H, batch, inverse, scale)
+
+
+__all__ += ["fft16", "fft2_16"]

@@ -31,6 +31,10 @@ GFFT_C2R = 6
 
 GFFT_F32 = 0
 GFFT_F64 = 1
+GFFT_F16 = 2
+GFFT_BF16 = 3
+GFFT_SCALE_GIVEN = 0
+GFFT_SCALE_AUTO = 1
 GFFT_FLAG_HALF = 1
 GFFT_FORWARD = 0
 GFFT_INVERSE = 1
@@ -41,6 +45,7 @@ EXPORTS = (
     "gfft_plan_destroy",
     "gfft_execute",
     "gfft_execute_host",
+    "gfft_execute_scaled",
     "gfft_plan_workspace",
     "gfft_plan_describe",
     "gfft_plan_explain",
@@ -113,6 +118,8 @@ def lib() -> ctypes.CDLL:
         L.gfft_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(GfftDesc)]
         L.gfft_plan_create.restype = i32
         L.gfft_plan_destroy.argtypes = [vp]
+        L.gfft_execute_scaled.argtypes = [vp, vp, vp, i32, vp, i32, vp]
+        L.gfft_execute_scaled.restype = i32
         L.gfft_plan_destroy.restype = i32
         L.gfft_execute.argtypes = [vp, vp, vp, i32, vp]
         L.gfft_execute.restype = i32

@@ -177,6 +177,12 @@ struct TileArgs {
     const void *pass_tw;  // Stockham pass twiddles, complex T
     const double *tw_lo;  // complex double [1 << tw_lo_bits]: W_Ntw^m
     const double *tw_hi;  // complex double [Ntw >> tw_lo_bits]: W_Ntw^(m << tw_lo_bits)
+    // Per-transform output scale (16-bit storage kernels): sequence starting
+    // at element offset q of this launch's input belongs to transform
+    // (scale_off + q) / scale_div, and its stored values are multiplied by
+    // scale[] of that transform.  Null: no scaling.
+    const float *scale;
+    int64_t scale_div, scale_off;
 };
 
 // FP64 twiddle W_Ntw^m from the two-level table.
@@ -479,6 +485,147 @@ __device__ __forceinline__ void store_tile(const cx<T> *lds, const TileArgs &a, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// 16-bit storage (FP16 / BF16): a complex element is one 32-bit word, the
+// real part in the low half.  Values are widened to FP32 on load, all
+// arithmetic runs in FP32 in LDS, and they are rounded once (to nearest
+// even) on store.
+// ---------------------------------------------------------------------------
+struct st_f16 {};
+struct st_bf16 {};
+
+template <class S>
+struct HalfFmt;
+template <>
+struct HalfFmt<st_f16> {
+    static __device__ __forceinline__ float dec(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
+    static __device__ __forceinline__ uint32_t enc(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
+};
+template <>
+struct HalfFmt<st_bf16> {
+    static __device__ __forceinline__ float dec(uint32_t h) { return __uint_as_float(h << 16); }
+    // round to nearest even on the 16 dropped bits; a NaN stays a (quiet) NaN
+    static __device__ __forceinline__ uint32_t enc(float f)
+    {
+        const uint32_t u = __float_as_uint(f);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+        return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    }
+};
+template <class S>
+__device__ __forceinline__ cx<float> hdec(uint32_t w)
+{
+    return {HalfFmt<S>::dec(w & 0xffffu), HalfFmt<S>::dec(w >> 16)};
+}
+template <class S>
+__device__ __forceinline__ uint32_t henc(cx<float> v)
+{
+    return HalfFmt<S>::enc(v.re) | (HalfFmt<S>::enc(v.im) << 16);
+}
+
+// load_tile for 16-bit storage: 16-byte vectors carry four elements of one
+// sequence (rows) or of four adjacent sequences (columns).
+template <class S, int L, int M, int NT, class G, int INP>
+__device__ __forceinline__ void load_tile_h(cx<float> *lds, const TileArgs &a, int o2, int o, int s0, int nvalid,
+                                            int tid)
+{
+    constexpr int TOT = M * L;
+    const bool sw = swap_in_of(a, o);
+    const uint32_t *in = reinterpret_cast<const uint32_t *>(a.in) + in_base(a, o2, o);
+    if constexpr (INP == PAT_ROW) in += (int64_t)s0 * a.in_inner;
+    else in += s0;
+    if constexpr (TOT % (4 * NT) == 0 && (INP == PAT_ROW ? L : M) % 4 == 0) {
+        if (a.vec_in && nvalid == M) {
+#pragma unroll
+            for (int it = 0; it < TOT / (4 * NT); it++) {
+                const int e = (tid + it * NT) * 4;
+                if constexpr (INP == PAT_ROW) {
+                    const int s = e / L, n = e % L;
+                    const uint4 q = *reinterpret_cast<const uint4 *>(in + (int64_t)s * a.in_inner + n);
+                    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const cx<float> x = hdec<S>(w[v]);
+                        lds[G::off(s, n + v)] = sw ? swap_ri(x) : x;
+                    }
+                } else {
+                    const int n = e / M, s = e % M;
+                    const uint4 q = *reinterpret_cast<const uint4 *>(in + (int64_t)n * a.in_elem + s);
+                    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const cx<float> x = hdec<S>(w[v]);
+                        lds[G::off(s + v, n)] = sw ? swap_ri(x) : x;
+                    }
+                }
+            }
+            return;
+        }
+    }
+#pragma unroll 4
+    for (int it = 0; it < (TOT + NT - 1) / NT; it++) {
+        const int e = tid + it * NT;
+        if (TOT % NT == 0 || e < TOT) {
+            const int s = INP == PAT_ROW ? e / L : e % M;
+            const int n = INP == PAT_ROW ? e % L : e / M;
+            if (s < nvalid) {
+                const cx<float> x =
+                    hdec<S>(INP == PAT_ROW ? in[(int64_t)s * a.in_inner + n] : in[(int64_t)n * a.in_elem + s]);
+                lds[G::off(s, n)] = sw ? swap_ri(x) : x;
+            }
+        }
+    }
+}
+
+// store_tile for 16-bit storage; `scl` (LDS, one factor per sequence of the
+// tile) multiplies the values before rounding when non-null.
+template <class S, int L, int M, int NT, class G, int OUTP>
+__device__ __forceinline__ void store_tile_h(const cx<float> *lds, const TileArgs &a, int o2, int o, int s0,
+                                             int nvalid, int tid, const float *scl)
+{
+    constexpr int TOT = M * L;
+    const bool sw = swap_out_of(a, o);
+    uint32_t *out = reinterpret_cast<uint32_t *>(a.out) + out_base(a, o2, o);
+    if constexpr (OUTP == PAT_ROW) out += (int64_t)s0 * a.out_inner;
+    else out += s0;
+    auto get = [&](int s, int k) {
+        cx<float> x = lds[G::off(s, k)];
+        if (scl) x = {x.re * scl[s], x.im * scl[s]};
+        return henc<S>(sw ? swap_ri(x) : x);
+    };
+    if constexpr (TOT % (4 * NT) == 0 && (OUTP == PAT_ROW ? L : M) % 4 == 0) {
+        if (a.vec_out && nvalid == M) {
+#pragma unroll
+            for (int it = 0; it < TOT / (4 * NT); it++) {
+                const int e = (tid + it * NT) * 4;
+                if constexpr (OUTP == PAT_ROW) {
+                    const int s = e / L, k = e % L;
+                    *reinterpret_cast<uint4 *>(out + (int64_t)s * a.out_inner + k) =
+                        uint4{get(s, k), get(s, k + 1), get(s, k + 2), get(s, k + 3)};
+                } else {
+                    const int k = e / M, s = e % M;
+                    *reinterpret_cast<uint4 *>(out + (int64_t)k * a.out_elem + s) =
+                        uint4{get(s, k), get(s + 1, k), get(s + 2, k), get(s + 3, k)};
+                }
+            }
+            return;
+        }
+    }
+#pragma unroll 4
+    for (int it = 0; it < (TOT + NT - 1) / NT; it++) {
+        const int e = tid + it * NT;
+        if (TOT % NT == 0 || e < TOT) {
+            const int s = OUTP == PAT_ROW ? e / L : e % M;
+            const int k = OUTP == PAT_ROW ? e % L : e / M;
+            if (s < nvalid) {
+                const uint32_t w = get(s, k);
+                if (OUTP == PAT_ROW) out[(int64_t)s * a.out_inner + k] = w;
+                else out[(int64_t)k * a.out_elem + s] = w;
+            }
+        }
+    }
+}
+
 // LDS geometry of a tile: element n of sequence s lives at G::off(s, n).
 //   LAY_SEQ : sequence-major.  One pad element after every 16 elements of a
 //             sequence breaks the stride-R conflicts of the Stockham writes;
@@ -529,6 +676,37 @@ __global__ void __launch_bounds__(NT) tile_fft_kernel(TileArgs a)
     stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, false, 1, RS...>(
         lds, reinterpret_cast<const cx<T> *>(a.pass_tw), tid, a, s0, o, Tw4Pre{});
     store_tile<T, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, tid);
+}
+
+// tile_fft_kernel over 16-bit storage (FP32 arithmetic).  The launch that
+// reads the plan's input applies the per-transform scale on its store.
+template <class S, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
+__global__ void __launch_bounds__(NT) tile_fft_kernel_h(TileArgs a)
+{
+    using T = float;
+    using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
+    __shared__ cx<T> lds[G::ELEMS];
+    __shared__ float scl[M];
+    const int tid = threadIdx.x;
+    const int tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int t2 = tile / a.tiles_inner;
+    const int ti = tile - t2 * a.tiles_inner;
+    const int o2 = t2 / a.n_outer1;
+    const int o = t2 - o2 * a.n_outer1;
+    const int s0 = ti * M;
+    const int nvalid = min(M, a.n_inner - s0);
+    load_tile_h<S, L, M, NT, G, INP>(lds, a, o2, o, s0, nvalid, tid);
+    if (a.scale) {
+        const int64_t base = in_base(a, o2, o);
+        for (int s = tid; s < nvalid; s += NT) {
+            const int64_t q = base + (INP == PAT_ROW ? (int64_t)(s0 + s) * a.in_inner : (int64_t)(s0 + s));
+            scl[s] = a.scale[(a.scale_off + q) / a.scale_div];
+        }
+    }
+    __syncthreads();
+    stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, false, 1, RS...>(
+        lds, reinterpret_cast<const cx<T> *>(a.pass_tw), tid, a, s0, o, Tw4Pre{});
+    store_tile_h<S, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, tid, a.scale ? scl : nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -58,6 +58,23 @@ struct CopyArgs {
     int32_t real_in, real_out;
 };
 
+// Per-transform scale of a 16-bit storage transform: scale[b] = 2^-k, the
+// largest power of two <= 1 with gain * max|x_b|_1 * 2^-k <= limit, where
+// max|x|_1 is the largest |re| + |im| of transform b's input (rows x width
+// elements, row pitch `pitch`, transform b at b * dist).  Every partial and
+// final DFT value of the transform is bounded by gain * max|x|_1.
+struct AmaxArgs {
+    const void *in;   // 16-bit pairs
+    uint32_t *bits;   // batch words: running max (float bits), then the scale (float)
+    int32_t store;    // 1 FP16, 2 BF16
+    int32_t vec;      // 16-byte loads (set by the launcher)
+    int64_t batch, rows, width, pitch, dist;
+    int64_t chunk, chunks;  // elements per block and blocks per transform (set by the launcher)
+    float gain, limit;
+};
+
+hipError_t launch_amax_scale(AmaxArgs a, hipStream_t s);
+
 hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s);
 hipError_t launch_pack(int prec, const UnpackArgs &a, hipStream_t s);
 hipError_t launch_separate(int prec, const SeparateArgs &a, hipStream_t s);

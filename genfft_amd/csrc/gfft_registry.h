@@ -4,6 +4,8 @@
 // the planner (plan.cpp) picks entries by (precision, L, input pattern,
 // output pattern) and launches them through the function pointer.
 #pragma once
+#include <type_traits>
+
 #include "gfft_device.h"
 
 namespace gfft {
@@ -21,6 +23,7 @@ struct KernelEntry {
     int persistent;  // 1: tile_fft_pipe_kernel (grid sized to the device, loops over tiles)
     int tune_only;   // 1: an alternative (radix order / tile width / threads) only the autotuner selects
     void (*fn)(TileArgs);
+    int store;       // 0: complex T in memory; 1: FP16 pairs; 2: BF16 pairs (FP32 arithmetic)
 };
 
 template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
@@ -72,8 +75,21 @@ constexpr KernelEntry make_pipe_entry()
     return e;
 }
 
-// Defined in kern_tile_f32.hip / kern_tile_f64.hip.
+// 16-bit storage form of a complex64 shape (tile_fft_kernel_h).
+template <class S, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
+constexpr KernelEntry make_entry_h()
+{
+    KernelEntry e = make_entry<float, L, M, NT, LAY, INP, OUTP, RS...>();
+    e.lds_bytes += M * (int)sizeof(float);
+    e.store = std::is_same<S, st_f16>::value ? 1 : 2;
+    e.fn = &tile_fft_kernel_h<S, L, M, NT, LAY, INP, OUTP, RS...>;
+    return e;
+}
+
+// Defined in kern_tile_f32.hip / kern_tile_f64.hip / kern_tile_h.hip.
 const KernelEntry *tile_kernels_f32(int *count);
 const KernelEntry *tile_kernels_f64(int *count);
+const KernelEntry *tile_kernels_f16(int *count);
+const KernelEntry *tile_kernels_bf16(int *count);
 
 }  // namespace gfft

@@ -9,6 +9,7 @@
 //   interleave  transform_interleave's packing (include/genFFT/fft.h:100-105).
 //   hfill       RealFFT2D's Hermitian column fill (include/genFFT/FFTReal.h:90-103).
 //   copy        the n = 1 transform (identity on each element).
+//   amax        per-transform power-of-two scale of 16-bit storage input.
 //
 // All of them are grid-stride loops over (pair, batch) with 64-bit indices
 // and read each input element once.
@@ -190,6 +191,86 @@ __global__ void __launch_bounds__(256) copy_kernel(CopyArgs a)
     }
 }
 
+// ---- per-transform scale of 16-bit storage transforms -------------------
+__device__ __forceinline__ float scale_from_max(float m, float gain, float limit)
+{
+    const float q = m * gain / limit;  // limit is a power of two: exact division
+    if (!(q > 1.0f) || !(q <= 3.0e38f)) return 1.0f;  // fits already, or not finite
+    const uint32_t u = __float_as_uint(q);
+    const int k = (int)((u >> 23) & 255u) - 127 + ((u & 0x7fffffu) ? 1 : 0);  // ceil(log2 q), q normal
+    return __uint_as_float((uint32_t)(127 - k) << 23);
+}
+
+template <class S>
+__device__ __forceinline__ float l1_of(uint32_t w)
+{
+    const cx<float> v = hdec<S>(w);
+    return fabsf(v.re) + fabsf(v.im);
+}
+
+// small transforms (rows * width <= 1024): one thread per transform writes
+// its scale directly
+template <class S>
+__global__ void __launch_bounds__(256) amax_small_kernel(AmaxArgs a)
+{
+    const uint32_t *in = reinterpret_cast<const uint32_t *>(a.in);
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < a.batch;
+         b += (int64_t)gridDim.x * blockDim.x) {
+        float m = 0.0f;
+        for (int64_t r = 0; r < a.rows; r++)
+            for (int64_t w = 0; w < a.width; w++) m = fmaxf(m, l1_of<S>(in[b * a.dist + r * a.pitch + w]));
+        reinterpret_cast<float *>(a.bits)[b] = scale_from_max(m, a.gain, a.limit);
+    }
+}
+
+__global__ void __launch_bounds__(256) amax_reset_kernel(AmaxArgs a)
+{
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < a.batch;
+         b += (int64_t)gridDim.x * blockDim.x)
+        a.bits[b] = 0u;
+}
+
+// block = `chunk` elements of one transform; max over the block, then one
+// atomic max per block on the float bits (non-negative floats order like
+// their bit patterns, so the result does not depend on the order)
+template <class S>
+__global__ void __launch_bounds__(256) amax_kernel(AmaxArgs a)
+{
+    __shared__ float red[4];
+    const uint32_t *in = reinterpret_cast<const uint32_t *>(a.in);
+    const int64_t b = blockIdx.x / a.chunks, c = blockIdx.x - b * a.chunks;
+    const int64_t E = a.rows * a.width;
+    const int64_t e0 = c * a.chunk, e1 = e0 + a.chunk < E ? e0 + a.chunk : E;
+    const uint32_t *base = in + b * a.dist;
+    float m = 0.0f;
+    if (a.vec) {  // width, pitch, dist multiples of 4 and 16-byte aligned input
+        for (int64_t e = e0 + 4 * (int64_t)threadIdx.x; e < e1; e += 4 * 256) {
+            const int64_t r = e / a.width, w = e - r * a.width;
+            const uint4 q = *reinterpret_cast<const uint4 *>(base + r * a.pitch + w);
+            m = fmaxf(m, fmaxf(fmaxf(l1_of<S>(q.x), l1_of<S>(q.y)), fmaxf(l1_of<S>(q.z), l1_of<S>(q.w))));
+        }
+    } else {
+        for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+            const int64_t r = e / a.width, w = e - r * a.width;
+            m = fmaxf(m, l1_of<S>(base[r * a.pitch + w]));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        atomicMax(a.bits + b, __float_as_uint(m));
+    }
+}
+
+__global__ void __launch_bounds__(256) amax_finish_kernel(AmaxArgs a)
+{
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < a.batch;
+         b += (int64_t)gridDim.x * blockDim.x)
+        reinterpret_cast<float *>(a.bits)[b] = scale_from_max(__uint_as_float(a.bits[b]), a.gain, a.limit);
+}
+
 unsigned grid_for(int64_t work)
 {
     int64_t blocks = (work + 255) / 256;
@@ -199,6 +280,28 @@ unsigned grid_for(int64_t work)
 }
 
 }  // namespace
+
+hipError_t launch_amax_scale(AmaxArgs a, hipStream_t s)
+{
+    const int64_t E = a.rows * a.width;
+    const bool f16 = a.store == 1;
+    if (E <= 1024) {
+        const unsigned g = grid_for(a.batch);
+        if (f16) hipLaunchKernelGGL(amax_small_kernel<st_f16>, dim3(g), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(amax_small_kernel<st_bf16>, dim3(g), dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
+    a.chunk = 8192;
+    a.chunks = (E + a.chunk - 1) / a.chunk;
+    if (a.batch * a.chunks > 0x7fffffffll) return hipErrorInvalidValue;
+    a.vec = (a.width % 4 == 0) && (a.pitch % 4 == 0) && (a.dist % 4 == 0) && (((uintptr_t)a.in & 15) == 0);
+    const unsigned g = grid_for(a.batch);
+    hipLaunchKernelGGL(amax_reset_kernel, dim3(g), dim3(256), 0, s, a);
+    if (f16) hipLaunchKernelGGL(amax_kernel<st_f16>, dim3((unsigned)(a.batch * a.chunks)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(amax_kernel<st_bf16>, dim3((unsigned)(a.batch * a.chunks)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(amax_finish_kernel, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s)
 {

@@ -238,11 +238,18 @@ struct PlanOpts {
 // Picks the tile kernel for (precision, L, patterns): the registry entry
 // `variant` when it matches, else the widest tile (or the requested M) in its
 // persistent pipelined form when one is compiled and allowed by pipe_mask.
+const KernelEntry *tile_table(int prec, int store, int *cnt)
+{
+    if (store == 1) return tile_kernels_f16(cnt);
+    if (store == 2) return tile_kernels_bf16(cnt);
+    return prec ? tile_kernels_f64(cnt) : tile_kernels_f32(cnt);
+}
+
 const KernelEntry *find_tile(int prec, int L, int inp, int outp, int prefer_m = 0, int pipe_mask = 3,
-                             int variant = -1)
+                             int variant = -1, int store = 0)
 {
     int cnt = 0;
-    const KernelEntry *tab = prec ? tile_kernels_f64(&cnt) : tile_kernels_f32(&cnt);
+    const KernelEntry *tab = tile_table(prec, store, &cnt);
     if (variant >= 0 && variant < cnt && tab[variant].L == L && tab[variant].inp == inp && tab[variant].outp == outp)
         return &tab[variant];
     const bool pipe = (inp == PAT_COL ? (pipe_mask & 1) : (pipe_mask & 2)) != 0;
@@ -261,10 +268,10 @@ const KernelEntry *find_tile(int prec, int L, int inp, int outp, int prefer_m = 
     return best;
 }
 
-int max_len(int prec, int inp, int outp)
+int max_len(int prec, int inp, int outp, int store = 0)
 {
     int cnt = 0, best = 0;
-    const KernelEntry *tab = prec ? tile_kernels_f64(&cnt) : tile_kernels_f32(&cnt);
+    const KernelEntry *tab = tile_table(prec, store, &cnt);
     for (int i = 0; i < cnt; i++)
         if (tab[i].inp == inp && tab[i].outp == outp) best = std::max(best, tab[i].L);
     return best;
@@ -337,6 +344,9 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
 enum BufId { B_IN = 0, B_OUT = 1, B_SCR = 2 };
 enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET };
 
+// bytes of one complex element in memory
+size_t csize_of(int prec, int store) { return store ? 4 : (prec ? 16 : 8); }
+
 struct Launch {
     LaunchType type;
     const KernelEntry *k = nullptr;
@@ -357,8 +367,14 @@ struct Launch {
 
 struct gfft_plan_s {
     gfft_desc d{};
-    int prec = 0;
+    int prec = 0;   // arithmetic: 0 FP32, 1 FP64
+    int store = 0;  // memory format: 0 complex of the arithmetic type, 1 FP16 pairs, 2 BF16 pairs
     int device = 0;
+    // per-transform scale (16-bit storage): transform b starts at element
+    // b * scale_div of the input; the auto-scale pass reads its rows x width
+    // elements (row pitch amax_pitch) and bounds every stored value by
+    // amax_gain * max(|re| + |im|)
+    int64_t scale_div = 1, amax_rows = 1, amax_width = 1, amax_pitch = 1, amax_gain = 1;
     std::vector<gfft::Launch> launches;
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -382,6 +398,7 @@ struct Builder {
     const PlanOpts *forced = nullptr;  // autotuner candidate (overrides table and environment)
     PlanOpts opts = resolve_opts(prec, 0, forced);
     int64_t scratch_elems = 0;  // complex elements of scratch needed
+    int store = 0;              // 16-bit storage format (tile_kernels_f16 / _bf16), default shapes only
 
     void need_scratch(int64_t elems) { scratch_elems = std::max(scratch_elems, elems); }
 
@@ -392,7 +409,7 @@ struct Builder {
              int64_t out_outer2, int64_t out_outer, int64_t out_inner, int64_t out_elem,
              int tw_mode, int64_t tw_n, bool first, bool last, int variant = -1)
     {
-        const KernelEntry *k = find_tile(prec, L, inp, outp, prefer_m, opts.pipe_mask, variant);
+        const KernelEntry *k = find_tile(prec, L, inp, outp, prefer_m, opts.pipe_mask, variant, store);
         if (!k) return fail(GFFT_ERR_UNSUPPORTED, "no tile kernel for L=%d pattern %d->%d", L, inp, outp);
         Launch l;
         l.type = LT_TILE;
@@ -432,11 +449,17 @@ struct Builder {
         if (st) return st;
         // 16-byte vectors need even element offsets everywhere in the tile
         auto even = [](int64_t v) { return (v & 1) == 0; };
-        if (prec == 0) {
+        if (prec == 0 && store == 0) {
             l.vin_ok = even(in_outer2) && even(in_outer) && even(in_off_el) &&
                        (inp == PAT_ROW ? even(in_inner) : even(in_elem));
             l.vout_ok = even(out_outer2) && even(out_outer) && even(out_off_el) &&
                         (outp == PAT_ROW ? even(out_inner) : even(out_elem));
+        } else if (store) {
+            // 16-bit pairs: a 16-byte vector is four elements
+            auto q4 = [](int64_t v) { return (v & 3) == 0; };
+            l.vin_ok = q4(in_outer2) && q4(in_outer) && q4(in_off_el) && (inp == PAT_ROW ? q4(in_inner) : q4(in_elem));
+            l.vout_ok = q4(out_outer2) && q4(out_outer) && q4(out_off_el) &&
+                        (outp == PAT_ROW ? q4(out_inner) : q4(out_elem));
         }
         l.in_buf = in_buf;
         l.out_buf = out_buf;
@@ -477,9 +500,9 @@ struct Builder {
             p->launches.push_back(l);
             return GFFT_OK;
         }
-        const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
+        const int maxrow = max_len(prec, PAT_ROW, PAT_ROW, store);
         const PlanOpts saved = opts;
-        opts = resolve_opts(prec, n, forced);
+        opts = store ? PlanOpts{} : resolve_opts(prec, n, forced);
         struct Restore {
             PlanOpts &dst;
             PlanOpts v;
@@ -493,7 +516,7 @@ struct Builder {
         int l1, l2;
         four_step_split(prec, lg, &l1, &l2, opts.n1_log2);
         const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
-        if (N1 > max_len(prec, PAT_COL, PAT_COL) || N2 > max_len(prec, PAT_ROW, PAT_COL))
+        if (N1 > max_len(prec, PAT_COL, PAT_COL, store) || N2 > max_len(prec, PAT_ROW, PAT_COL, store))
             return fail(GFFT_ERR_UNSUPPORTED, "length %lld exceeds the two-pass limit", (long long)n);
         // Measured on MI355X (c64 2^20 x 128): one launch per pass over the
         // whole batch beats chunking through a small scratch ring (launch
@@ -538,7 +561,7 @@ struct Builder {
             }
             return GFFT_OK;
         }
-        const int maxcol = max_len(prec, PAT_COL, PAT_COL);
+        const int maxcol = max_len(prec, PAT_COL, PAT_COL, store);
         if (rows <= maxcol)
             return tile((int)rows, PAT_COL, PAT_COL, 0, in_buf, 0, out_buf, 0, cols, batch, 1, 0, in_dist, 1,
                         in_pitch, 0, out_dist, 1, out_pitch, TW_NONE, 0, first, last);
@@ -584,13 +607,17 @@ int64_t dflt(int64_t v, int64_t d) { return v ? v : d; }
 int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
 {
     const gfft_desc &d = p->d;
-    Builder b{p, p->prec, (size_t)(p->prec ? 16 : 8), dry, forced};
+    Builder b{p, p->prec, csize_of(p->prec, p->store), dry, forced};
+    b.store = p->store;
+    if (p->store) b.opts = PlanOpts{};
     const size_t cs = b.csize, rs = cs / 2;
     const int64_t n = d.n, batch = d.batch;
     int st = GFFT_OK;
     switch (d.kind) {
     case GFFT_C2C: {
         const int64_t id = dflt(d.in_dist, n), od = dflt(d.out_dist, n);
+        p->scale_div = id;
+        p->amax_width = p->amax_pitch = p->amax_gain = n;
         p->in_bytes = ((batch - 1) * id + n) * cs;
         p->out_bytes = ((batch - 1) * od + n) * cs;
         p->out_has_gaps = od > n;
@@ -664,6 +691,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         const int64_t ip = dflt(d.in_stride, cols), op = dflt(d.out_stride, cols);
         if (ip < cols || op < cols) return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the column count");
         const int64_t id = dflt(d.in_dist, rows * ip), od = dflt(d.out_dist, rows * op);
+        p->scale_div = id;
+        p->amax_rows = p->amax_gain = rows;
+        p->amax_width = cols;
+        p->amax_pitch = ip;
         p->in_bytes = ((batch - 1) * id + (rows - 1) * ip + cols) * cs;
         p->out_bytes = ((batch - 1) * od + (rows - 1) * op + cols) * cs;
         p->out_has_gaps = op > cols || od > rows * op;
@@ -675,10 +706,15 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         const int64_t ip = dflt(d.in_stride, W), op = dflt(d.out_stride, W);
         if (ip < W || op < W) return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the width");
         const int64_t id = dflt(d.in_dist, H * ip), od = dflt(d.out_dist, H * op);
+        p->scale_div = id;
+        p->amax_rows = H;
+        p->amax_width = W;
+        p->amax_pitch = ip;
+        p->amax_gain = W * H;
         p->in_bytes = ((batch - 1) * id + (H - 1) * ip + W) * cs;
         p->out_bytes = ((batch - 1) * od + (H - 1) * op + W) * cs;
         p->out_has_gaps = op > W || od > H * op;
-        if (W > max_len(p->prec, PAT_ROW, PAT_ROW))
+        if (W > max_len(p->prec, PAT_ROW, PAT_ROW, p->store))
             return fail(GFFT_ERR_UNSUPPORTED, "2D width %lld exceeds the single-pass row limit", (long long)W);
         if (W == 1) {
             st = b.columns(H, 1, batch, B_IN, ip, id, B_OUT, op, od, true, true);
@@ -745,7 +781,7 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
 // A planner mistake becomes an error here instead of a device fault.
 int validate(const gfft_plan_s *p)
 {
-    const int64_t cs = p->prec ? 16 : 8, rs = cs / 2;
+    const int64_t cs = (int64_t)csize_of(p->prec, p->store), rs = cs / 2;
     const int64_t ext[3] = {(int64_t)p->in_bytes, (int64_t)p->out_bytes, (int64_t)p->scratch_bytes};
     auto inside = [&](int buf, int64_t off, int64_t last_byte, const char *what) {
         if (off < 0 || last_byte < 0 || off + last_byte > ext[buf])
@@ -875,17 +911,22 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
 
 // Executes the plan's launches in order on stream s.  With `events`, records
 // events[2i] / events[2i+1] on s around launch i (per-kernel timing).
+// With `scale` (16-bit storage plans), the tile launches that read the plan's
+// input multiply transform b's values by scale[b] on their store.
 int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, const void *in, void *out,
-                 void *scratch, int direction, hipStream_t s, hipEvent_t *events = nullptr);
+                 void *scratch, int direction, hipStream_t s, hipEvent_t *events = nullptr,
+                 const float *scale = nullptr, int64_t scale_div = 1);
 
 int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
-        hipEvent_t *events = nullptr)
+        hipEvent_t *events = nullptr, const float *scale = nullptr)
 {
-    return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events);
+    return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events, scale,
+                        p->scale_div);
 }
 
 int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, const void *in, void *out,
-                 void *scratch, int direction, hipStream_t s, hipEvent_t *events)
+                 void *scratch, int direction, hipStream_t s, hipEvent_t *events, const float *scale,
+                 int64_t scale_div)
 {
     struct {
         int prec, xcd_remap;
@@ -908,6 +949,11 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             a.vec_in = l.vin_ok && (((uintptr_t)ip & 15) == 0);
             a.vec_out = l.vout_ok && (((uintptr_t)op & 15) == 0);
             a.xcd_remap = p->xcd_remap;
+            if (scale && l.in_buf == B_IN) {
+                a.scale = scale;
+                a.scale_div = scale_div;
+                a.scale_off = l.in_off / (int64_t)csize_of(l.k->prec, l.k->store);
+            }
             void *args[] = {&a};
             HIP_TRY(hipLaunchKernel((const void *)l.k->fn, dim3((unsigned)l.grid), dim3(l.k->NT), args, 0, s));
             break;
@@ -1387,8 +1433,8 @@ int gfft_is_device_pointer(const void *ptr)
 
 static int check_desc(const gfft_desc &d)
 {
-    if (d.precision != GFFT_F32 && d.precision != GFFT_F64)
-        return fail(GFFT_ERR_INVALID_ARG, "precision must be GFFT_F32 or GFFT_F64");
+    if (d.precision != GFFT_F32 && d.precision != GFFT_F64 && d.precision != GFFT_F16 && d.precision != GFFT_BF16)
+        return fail(GFFT_ERR_INVALID_ARG, "precision must be GFFT_F32, GFFT_F64, GFFT_F16 or GFFT_BF16");
     if (d.kind < GFFT_C2C || d.kind > GFFT_C2R) return fail(GFFT_ERR_INVALID_ARG, "unknown kind %d", d.kind);
     if (d.batch < 1) return fail(GFFT_ERR_INVALID_ARG, "batch must be >= 1");
     if (!is_pow2(d.n) || d.n > ((int64_t)1 << 23))
@@ -1403,7 +1449,18 @@ static int check_desc(const gfft_desc &d)
     const int64_t width = d.kind == GFFT_COLS ? d.n2 : d.n;
     if (two && ((d.in_stride && d.in_stride < width) || (d.out_stride && d.out_stride < width)))
         return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the row length");
+    if (d.precision == GFFT_F16 || d.precision == GFFT_BF16) {
+        if (d.kind != GFFT_C2C && d.kind != GFFT_C2C_2D && d.kind != GFFT_COLS)
+            return fail(GFFT_ERR_UNSUPPORTED, "16-bit storage supports C2C, C2C_2D and COLS plans");
+        if (d.n < 2) return fail(GFFT_ERR_UNSUPPORTED, "16-bit storage needs a transform length >= 2");
+    }
     return GFFT_OK;
+}
+
+static void set_precision(gfft_plan_s *p, int precision)
+{
+    p->prec = precision == GFFT_F64 ? 1 : 0;
+    p->store = precision == GFFT_F16 ? 1 : precision == GFFT_BF16 ? 2 : 0;
 }
 
 int gfft_plan_explain(const gfft_desc *desc, char *buf, size_t len)
@@ -1413,7 +1470,7 @@ int gfft_plan_explain(const gfft_desc *desc, char *buf, size_t len)
     if (st) return st;
     std::unique_ptr<gfft_plan_s> p(new gfft_plan_s());
     p->d = *desc;
-    p->prec = desc->precision;
+    set_precision(p.get(), desc->precision);
     if ((st = build_plan(p.get(), true))) return st;
     if ((st = validate(p.get()))) return st;
     return describe(p.get(), buf, len);
@@ -1430,7 +1487,7 @@ int gfft_plan_create(gfft_plan *plan, const gfft_desc *desc)
     if (!device_ok(&dev)) return fail(GFFT_ERR_NO_DEVICE, "no HIP device available");
     std::unique_ptr<gfft_plan_s> p(new gfft_plan_s());
     p->d = d;
-    p->prec = d.precision;
+    set_precision(p.get(), d.precision);
     p->device = dev;
     p->xcd_remap = (int)env_int("GFFT_XCD_REMAP", 1);
     if (hipDeviceGetAttribute(&p->num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || p->num_cus < 1)
@@ -1466,6 +1523,36 @@ int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void 
         return fail(GFFT_ERR_INVALID_ARG, "this plan kind supports only the %s direction",
                     plan->dir_mask == 1 ? "forward" : "inverse");
     return run(plan, in, out, direction, (hipStream_t)stream);
+}
+
+int gfft_execute_scaled(gfft_plan plan, const void *in, void *out, int direction, float *scales, int scale_mode,
+                        void *stream)
+{
+    if (!plan) return fail(GFFT_ERR_INVALID_ARG, "null plan");
+    if (!in || !out || !scales) return fail(GFFT_ERR_INVALID_ARG, "null buffer");
+    if (direction != GFFT_FORWARD && direction != GFFT_INVERSE)
+        return fail(GFFT_ERR_INVALID_ARG, "direction must be GFFT_FORWARD or GFFT_INVERSE");
+    if (!plan->store) return fail(GFFT_ERR_UNSUPPORTED, "per-transform scales need a GFFT_F16 / GFFT_BF16 plan");
+    if (scale_mode != GFFT_SCALE_GIVEN && scale_mode != GFFT_SCALE_AUTO)
+        return fail(GFFT_ERR_INVALID_ARG, "scale_mode must be GFFT_SCALE_GIVEN or GFFT_SCALE_AUTO");
+    hipStream_t s = (hipStream_t)stream;
+    if (scale_mode == GFFT_SCALE_AUTO) {
+        AmaxArgs a{};
+        a.in = in;
+        a.bits = reinterpret_cast<uint32_t *>(scales);
+        a.store = plan->store;
+        a.batch = plan->d.batch;
+        a.rows = plan->amax_rows;
+        a.width = plan->amax_width;
+        a.pitch = plan->amax_pitch;
+        a.dist = plan->scale_div;
+        a.gain = (float)plan->amax_gain;
+        // largest bound a stored value may reach: 2^15 keeps FP16 (max 65504)
+        // finite with room for rounding; BF16 shares FP32's exponent range
+        a.limit = plan->store == 1 ? 32768.0f : 0x1p120f;
+        HIP_TRY(launch_amax_scale(a, s));
+    }
+    return run(plan, in, out, direction, s, nullptr, scales);
 }
 
 int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)

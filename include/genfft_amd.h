@@ -71,6 +71,15 @@ extern "C" {
 /* precisions */
 #define GFFT_F32 0      /* complex64  (interleaved float)  */
 #define GFFT_F64 1      /* complex128 (interleaved double) */
+/* 16-bit storage with FP32 arithmetic: a complex element is two 16-bit
+ * halves (real part first), 4 bytes; C2C, C2C_2D and COLS plans.  No
+ * reference counterpart (genFFT is float/double only). */
+#define GFFT_F16 2      /* IEEE binary16 pairs */
+#define GFFT_BF16 3     /* bfloat16 pairs      */
+
+/* gfft_execute_scaled scale modes */
+#define GFFT_SCALE_GIVEN 0 /* scales[b] supplied by the caller                      */
+#define GFFT_SCALE_AUTO 1  /* library writes scales[b] (power of two <= 1) first    */
 
 /* flags */
 #define GFFT_FLAG_HALF 1 /* R2C / DIT: write n/2+1 bins (else all n, Hermitian-filled) */
@@ -83,7 +92,7 @@ typedef struct gfft_plan_s *gfft_plan;
 
 typedef struct gfft_desc {
     int32_t kind;       /* GFFT_C2C ... GFFT_C2R                                   */
-    int32_t precision;  /* GFFT_F32 / GFFT_F64                                      */
+    int32_t precision;  /* GFFT_F32 / GFFT_F64 / GFFT_F16 / GFFT_BF16                */
     int64_t n;          /* 1D length; 2D width (columns); COLS column length (rows) */
     int64_t n2;         /* 2D height (rows); COLS number of columns; else ignored   */
     int64_t batch;      /* independent transforms (>= 1)                            */
@@ -111,6 +120,21 @@ GFFT_API int gfft_plan_destroy(gfft_plan plan);
  * only, C2R inverse only).  Calls that share a plan may not overlap in time
  * (they share the plan's workspace). */
 GFFT_API int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void *stream);
+
+/* gfft_execute for 16-bit storage plans with a per-transform scale:
+ * transform b (one batch member: a 1D transform, a 2D image, a COLS matrix)
+ * is stored as scales[b] * FFT(x_b), the factor applied in FP32 before the
+ * first rounding to 16 bits.  `scales` is a device array of `batch` floats.
+ * GFFT_SCALE_GIVEN reads it.  GFFT_SCALE_AUTO first writes it, on `stream`:
+ * scales[b] = the largest power of two <= 1 with
+ *     len_b * max_j(|Re x_bj| + |Im x_bj|) * scales[b] <= 2^15   (FP16)
+ * (len_b = the transform's point count, which bounds every partial and final
+ * DFT value), so no stored value overflows FP16's 65504; for BF16 the bound
+ * is 2^120 (FP32's exponent range), i.e. 1 for any finite input.  Powers of
+ * two are exact: divide the output by scales[b] to undo it.  No reference
+ * counterpart. */
+GFFT_API int gfft_execute_scaled(gfft_plan plan, const void *in, void *out, int direction, float *scales,
+                                 int scale_mode, void *stream);
 
 /* Same as gfft_execute for host buffers: stages through plan-owned device
  * memory and returns after the result is back in `out` (synchronous). */

@@ -174,3 +174,33 @@ def test_filter_plan_argument_checks():
     assert lib.gfft_filter_backward(None, None, None, None, None, None, None) == C.GFFT_ERR_INVALID_ARG
     if not torch.cuda.is_available():
         assert lib.gfft_filter_plan_create(ctypes.byref(h), C.GFFT_F32, 1024, 4) == C.GFFT_ERR_NO_DEVICE
+
+
+@pytest.mark.parametrize("prec", [C.GFFT_F16, C.GFFT_BF16])
+def test_16bit_storage_plans_in_bounds(prec):
+    """FP16 / BF16 storage: C2C 2..2^23, 2D (incl. 8192^2) and the COLS table
+    plan in bounds with four-byte elements, on the 16-bit kernel table."""
+    for k in range(1, 24):
+        n = 1 << k
+        for b in (1, 3, max(1, (1 << 28) // (n * 4))):
+            st, text = explain(desc(prec=prec, n=n, batch=b))
+            assert st == C.GFFT_OK, (n, b, C.lib().gfft_last_error())
+    st, text = explain(desc(prec=prec, n=1 << 20, batch=64))
+    assert "scratch_bytes=268435456" in text   # 2^20 x 64 four-byte elements
+    for w, h in ((8192, 8192), (16, 4096), (4096, 2), (2, 1)):
+        st, text = explain(desc(kind=C.GFFT_C2C_2D, prec=prec, n=w, n2=h))
+        assert st == C.GFFT_OK, (w, h, C.lib().gfft_last_error())
+    for n, cols in VERT_TABLE:
+        if n < 2:
+            continue
+        st, text = explain(desc(kind=C.GFFT_COLS, prec=prec, n=n, n2=cols, ins=cols + 3, outs=cols + 1))
+        assert st == C.GFFT_OK, (n, cols, C.lib().gfft_last_error())
+
+
+def test_16bit_storage_rejections():
+    lib = C.lib()
+    for kind in (C.GFFT_R2C, C.GFFT_C2R, C.GFFT_DIT, C.GFFT_R2C_2D):
+        assert explain(desc(kind=kind, prec=C.GFFT_F16, n=64, n2=8))[0] == C.GFFT_ERR_UNSUPPORTED
+    assert explain(desc(prec=C.GFFT_BF16, n=1))[0] == C.GFFT_ERR_UNSUPPORTED
+    assert explain(desc(prec=4, n=64))[0] == C.GFFT_ERR_INVALID_ARG
+    assert lib.gfft_execute_scaled(None, None, None, 0, None, 0, None) == C.GFFT_ERR_INVALID_ARG
