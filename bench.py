@@ -15,9 +15,18 @@ The JSON line also carries
   e2e           -- the whole transform against the HBM roofline;
   cpu_baseline  -- genFFT itself (oracle/_ref, built from the reference sources)
                    timed on the host cores over a bounded sample, or the
-                   oracle's C restatement when that library is absent.
+                   oracle's C restatement when that library is absent;
+  fused_filter  -- y = irfft(H * rfft(x)) on x [256, 16384] f32, H [8193]:
+                   fused forward + adjoint kernels against the same operator
+                   through unfused launches (rfft / irfft plans + elementwise);
+  uneven_batch  -- 20,000 complex64 transforms of sizes 2^6..2^16, mixed
+                   directions: one persistent launch replayed as a HIP graph
+                   against one plan launch per transform;
+  precision     -- N = 2^20 batch 64 with log-uniform magnitudes 1e-3..1e3:
+                   FP16 (+ per-transform scale), BF16 and FP32 modes, time and
+                   rel-L2 / max-abs against the GPU complex128 plan.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-extras]
 """
 import argparse
 import json
@@ -38,7 +47,12 @@ WORKLOADS = {
     "c64_n2^18_b512": ("c64", 1 << 18, 512),
     "c64_n2^22_b32": ("c64", 1 << 22, 32),
     "c128_n2^20_b64": ("c128", 1 << 20, 64),
+    # 16-bit storage (FP32 arithmetic), 4 bytes per complex element
+    "f16_n2^20_b64": ("f16", 1 << 20, 64),
+    "bf16_n2^20_b64": ("bf16", 1 << 20, 64),
+    "f16_n2^20_b256": ("f16", 1 << 20, 256),
 }
+CSIZE = {"c64": 8, "c128": 16, "f16": 4, "bf16": 4}
 
 
 def parse():
@@ -48,6 +62,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c64_n2^20_b128", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the fused-filter, uneven-batch and precision legs")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (scale reference for PMC byte counters)")
     return ap.parse_args()
@@ -107,6 +123,121 @@ def cpu_baseline(x_cpu, n, batch, prec, gpu_out_cpu):
     }
 
 
+def _time_ms(fn, reps, warm=2):
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_filter(dev):
+    """Fused spectral filter forward + adjoint vs the unfused launches."""
+    import torch
+
+    import genfft_amd as G
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.randn(256, 16384, device=dev, generator=g).requires_grad_(True)
+    H = (1 + 0.01 * torch.randn(8193, device=dev, generator=g)).to(torch.complex64).requires_grad_(True)
+    gy = torch.randn(256, 16384, device=dev, generator=g)
+    t = torch.roll(x.detach(), 3, -1) + 0.1 * torch.randn(256, 16384, device=dev, generator=g)
+    res = {}
+    for name, fn in (("fused", G.spectral_filter), ("unfused", G.spectral_filter_plain)):
+        def fwd_bwd():
+            x.grad = None
+            H.grad = None
+            fn(x, H).backward(gy)
+
+        def loss_step():
+            x.grad = None
+            H.grad = None
+            ((fn(x, H) - t) ** 2).mean().backward()
+        res[name] = {"fwd_ms": round(_time_ms(lambda: fn(x.detach(), H.detach()), 20), 4),
+                     "fwd_adj_ms": round(_time_ms(fwd_bwd, 20), 4),
+                     "loss_step_ms": round(_time_ms(loss_step, 20), 4)}
+    return {"config": "x [256,16384] f32 N(0,1), H [8193] complex64 = 1+0.01 N(0,1), "
+                      "loss mean|y-t|^2, t = shift(x,3)+0.1 N(0,1)",
+            "fused": res["fused"], "unfused": res["unfused"],
+            "speedup_fwd_adj": round(res["unfused"]["fwd_adj_ms"] / res["fused"]["fwd_adj_ms"], 2),
+            "speedup_fwd": round(res["unfused"]["fwd_ms"] / res["fused"]["fwd_ms"], 2),
+            "speedup_loss_step": round(res["unfused"]["loss_step_ms"] / res["fused"]["loss_step_ms"], 2)}
+
+
+def bench_uneven(dev):
+    """20,000 complex64 transforms, sizes 2^k with k uniform in 6..16, mixed directions."""
+    import numpy as np
+    import torch
+
+    import genfft_amd as G
+    rng = np.random.default_rng(0)
+    count = 20000
+    sizes = (1 << rng.integers(6, 17, count)).astype(np.int64)
+    inv = rng.integers(0, 2, count).astype(bool)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    total = int(sizes.sum())
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.complex(torch.rand(total, device=dev, generator=g) * 2 - 1,
+                      torch.rand(total, device=dev, generator=g) * 2 - 1)
+    y = torch.empty_like(x)
+    b = G.UnevenBatch(sizes, offs, offs, inv, torch.complex64)
+    ms = {m: _time_ms(lambda: b.execute(x, y, mode=m), r, 1) for m, r in
+          (("graph", 20), ("eager", 20), ("per_job", 3))}
+    return {"config": f"{count} complex64 transforms, sizes 2^6..2^16 (seed 0), 50/50 forward/inverse, "
+                      f"{total} elements", "graph_ms": round(ms["graph"], 4), "eager_ms": round(ms["eager"], 4),
+            "per_job_ms": round(ms["per_job"], 4),
+            "speedup_vs_per_job": round(ms["per_job"] / ms["graph"], 2),
+            "alg_gbs_graph": round(total * 16 / (ms["graph"] * 1e-3) / 1e9, 1)}
+
+
+def bench_precision(dev):
+    """N = 2^20 batch 64, log-uniform |x| in 1e-3..1e3, uniform phase: FP16 + scale, BF16, FP32."""
+    import math as m_
+
+    import torch
+
+    import genfft_amd as G
+    n, batch = 1 << 20, 64
+    g = torch.Generator(device=dev).manual_seed(20)
+    mag = torch.pow(10.0, torch.rand(batch, n, device=dev, generator=g, dtype=torch.float64) * 6 - 3)
+    z = torch.polar(mag, torch.rand(batch, n, device=dev, generator=g, dtype=torch.float64) * (2 * m_.pi))
+    del mag
+    out = {}
+    for mode in ("f16", "bf16", "f32"):
+        if mode == "f32":
+            x = z.to(torch.complex64)
+            xd = x.to(torch.complex128)
+            y = torch.empty_like(x)
+            plan = G.Plan(0, n, precision=0, batch=batch)
+            run = lambda: plan.execute(x, y)
+        else:
+            x = torch.view_as_real(z).to(torch.float16 if mode == "f16" else torch.bfloat16).contiguous()
+            xd = torch.view_as_complex(x.double().contiguous())
+            y = torch.empty_like(x)
+            sc = torch.empty(batch, dtype=torch.float32, device=dev)
+            plan = G.Plan(0, n, precision=2 if mode == "f16" else 3, batch=batch)
+            run = lambda: plan.execute_scaled(x, y, sc, True)
+        ms = _time_ms(run, 20)
+        run()
+        ref = G.fft(xd)
+        yd = y.to(torch.complex128) if mode == "f32" else \
+            torch.view_as_complex(y.double().contiguous()) / sc.double()[:, None]
+        err = yd - ref
+        out[mode] = {"ms": round(ms, 4), "gflops": round(5.0 * n * m_.log2(n) * batch / (ms * 1e-3) / 1e9, 1),
+                     "alg_gbs": round(2.0 * n * batch * (8 if mode == "f32" else 4) / (ms * 1e-3) / 1e9, 1),
+                     "rel_l2_vs_c128": float(torch.linalg.vector_norm(err) / torch.linalg.vector_norm(ref)),
+                     "max_abs_vs_c128": float(err.abs().max())}
+        del x, xd, y, ref, yd, err
+    out["note"] = ("FP16/BF16 include the automatic per-transform scale pass (one extra read); "
+                   "errors vs the FP64 long-double oracle: tests/test_gpu_precision.py")
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -120,15 +251,18 @@ def main():
     from genfft_amd import _capi as C
 
     prec, n, batch = WORKLOADS[a.workload]
-    cdt = torch.complex64 if prec == "c64" else torch.complex128
-    csize = 8 if prec == "c64" else 16
+    half = prec in ("f16", "bf16")
+    cdt = torch.complex128 if prec == "c128" else torch.complex64
+    csize = CSIZE[prec]
     dev = torch.device("cuda", 0)
     gen = torch.Generator(device=dev).manual_seed(0)
     # re/im i.i.d. N(0,1) from a fixed seed
     x = torch.complex(torch.randn(batch, n, device=dev, generator=gen, dtype=cdt.to_real()),
                       torch.randn(batch, n, device=dev, generator=gen, dtype=cdt.to_real()))
+    if half:
+        x = torch.view_as_real(x).to(torch.float16 if prec == "f16" else torch.bfloat16).contiguous()
     y = torch.empty_like(x)
-    plan = G.Plan(C.GFFT_C2C, n, precision=0 if prec == "c64" else 1, batch=batch)
+    plan = G.Plan(C.GFFT_C2C, n, precision={"c64": 0, "c128": 1, "f16": 2, "bf16": 3}[prec], batch=batch)
 
     if a.calib_copy:
         # a plain copy of known size: reads and writes exactly 1 GiB
@@ -193,8 +327,15 @@ def main():
             traffic = t.get("bytes_per_launch")
 
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and not half:
         cpu = cpu_baseline(x.cpu().numpy(), n, batch, prec, y.cpu().numpy())
+    del x, y, plan
+    torch.cuda.empty_cache()
+    extras = {}
+    if not a.no_extras and a.workload == "c64_n2^20_b128":
+        extras["fused_filter"] = bench_filter(dev)
+        extras["uneven_batch"] = bench_uneven(dev)
+        extras["precision"] = bench_precision(dev)
 
     line = {
         "metric": "batched c2c fp32 FFT GFLOP/s & %HBM peak at N=2^20; fused fwd+adjoint speedup",
@@ -207,9 +348,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if prec == "c64" else "f64",
+        "dtype": {"c64": "f32", "c128": "f64", "f16": "f16 storage, f32 math", "bf16": "bf16 storage, f32 math"}[prec],
         "data": "synthetic: re/im i.i.d. N(0,1), torch generator seed 0, resident in HBM",
-        "config": {"workload": f"batched 1D c2c forward, {'complex64' if prec == 'c64' else 'complex128'}, "
+        "config": {"workload": f"batched 1D c2c forward, "
+                               f"{ {'c64': 'complex64', 'c128': 'complex128', 'f16': 'FP16 pairs', 'bf16': 'BF16 pairs'}[prec]}, "
                                f"N=2^{int(math.log2(n))}, batch {batch}",
                    "n": n, "batch": batch, "precision": prec, "plan": [k for k in groups]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -222,6 +364,7 @@ def main():
                 "wall_s": round(wall, 3)},
         "cpu_baseline": cpu,
     }
+    line.update(extras)
     print(json.dumps(line), flush=True)
 
 

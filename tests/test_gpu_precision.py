@@ -190,3 +190,101 @@ def test_cols_16bit_odd_widths(gpu, fmt, rows, cols):
     ref = np.swapaxes(O.fft_ld(np.ascontiguousarray(np.swapaxes(_deq(x)[:, :cols], 0, 1))), 0, 1)
     assert O.rel_l2(yd[:, :cols], ref) <= _passes(p.describe()) * U[fmt]
     assert torch.all(out[:, cols:].float() == 7.0), "write into the pitch gap"
+
+
+# ---------------------------------------------------------------------------
+# The precision workload: N = 2^20 batch 64 and 2D 8192^2, inputs with
+# log-uniform magnitudes over 1e-3 .. 1e3 and uniform phase, in three modes
+# (FP16 + per-transform scale, BF16, full FP32).
+# ---------------------------------------------------------------------------
+def _log_uniform(shape, gpu, seed):
+    g = torch.Generator(device=gpu).manual_seed(seed)
+    mag = torch.pow(10.0, torch.rand(shape, device=gpu, generator=g, dtype=torch.float64) * 6 - 3)
+    ph = torch.rand(shape, device=gpu, generator=g, dtype=torch.float64) * (2 * np.pi)
+    return torch.polar(mag, ph)
+
+
+def _mode_input(z, mode):
+    """(what the transform reads, its exact complex128 value)"""
+    if mode == "f32":
+        x = z.to(torch.complex64)
+        return x, x.to(torch.complex128)
+    x = torch.view_as_real(z).to(TDT[mode]).contiguous()
+    return x, torch.view_as_complex(x.double().contiguous())
+
+
+def _mode_run(mode, x, two_d):
+    import genfft_amd as G
+    if mode == "f32":
+        y = G.fft2(x) if two_d else G.fft(x)
+        return y.to(torch.complex128), None
+    y, s = (G.fft2_16 if two_d else G.fft16)(x, scale="auto")
+    yd = torch.view_as_complex(y.double().contiguous())
+    shape = (-1,) + (1,) * (yd.dim() - 1)
+    return yd / s.double().view(shape), s
+
+
+def _gpu_err(a, b):
+    return ((torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item(), (a - b).abs().max().item())
+
+
+MODE_TOL = {"f16": 2 * U["f16"], "bf16": 2 * U["bf16"], "f32": O.tol_c64(1 << 20)}
+
+
+@pytest.mark.parametrize("mode", ["f16", "bf16", "f32"])
+def test_precision_workload_n2e20_b64(gpu, mode):
+    import genfft_amd as G
+    n, b = 1 << 20, 64
+    z = _log_uniform((b, n), gpu, 20)
+    x, xd = _mode_input(z, mode)
+    y, s = _mode_run(mode, x, False)
+    ref_g = G.fft(xd)
+    rg, mg = _gpu_err(y, ref_g)
+    rows = [0, b - 1]
+    ref_o = O.fft_ld(xd[rows].cpu().numpy())
+    yo = y[rows].cpu().numpy()
+    ro, mo = O.rel_l2(yo, ref_o), _max_abs(yo, ref_o)
+    print(f"precision N=2^20 b=64 {mode}: vs FP64 oracle (rows 0, 63) rel-L2 {ro:.3e} max-abs {mo:.3e}; "
+          f"vs GPU c128 rel-L2 {rg:.3e} max-abs {mg:.3e}"
+          + (f"; scales {s.min().item():g}..{s.max().item():g}" if s is not None else ""))
+    assert ro <= MODE_TOL[mode] and rg <= MODE_TOL[mode]
+    if mode == "f16":
+        # these inputs overflow FP16 without a scale (|X| reaches ~1e5 and more)
+        assert ref_g.abs().max().item() > 65504
+
+
+def _fft_ld_rows_mt(x: np.ndarray) -> np.ndarray:
+    from concurrent.futures import ThreadPoolExecutor
+    out = np.empty_like(x)
+    chunks = np.array_split(np.arange(x.shape[0]), 16)
+
+    def work(idx):
+        out[idx[0]:idx[-1] + 1] = O.fft_ld(x[idx[0]:idx[-1] + 1])
+
+    with ThreadPoolExecutor(16) as ex:
+        list(ex.map(work, chunks))
+    return out
+
+
+@pytest.mark.parametrize("mode", ["f16", "bf16", "f32"])
+def test_precision_workload_2d_8192(gpu, mode):
+    """8192 x 8192: against the GPU complex128 2D plan everywhere and the
+    long-double oracle on four whole output columns (rows transformed by the
+    oracle, then the four columns)."""
+    import genfft_amd as G
+    n = 8192
+    z = _log_uniform((1, n, n), gpu, 21)
+    x, xd = _mode_input(z, mode)
+    y, s = _mode_run(mode, x, True)
+    ref_g = G.fft2(xd)
+    rg, mg = _gpu_err(y, ref_g)
+    cols = [0, 1, 4097, 8191]
+    rows_spec = _fft_ld_rows_mt(xd[0].cpu().numpy())
+    ref_o = np.ascontiguousarray(O.fft_ld(np.ascontiguousarray(rows_spec[:, cols].T)).T)
+    yo = y[0][:, cols].cpu().numpy()
+    ro, mo = O.rel_l2(yo, ref_o), _max_abs(yo, ref_o)
+    print(f"precision 2D 8192^2 {mode}: vs FP64 oracle (4 columns) rel-L2 {ro:.3e} max-abs {mo:.3e}; "
+          f"vs GPU c128 rel-L2 {rg:.3e} max-abs {mg:.3e}"
+          + (f"; scale {s[0].item():g}" if s is not None else ""))
+    tol = {"f16": 3 * U["f16"], "bf16": 3 * U["bf16"], "f32": 2 * O.tol_c64(n * n)}[mode]
+    assert ro <= tol and rg <= tol
