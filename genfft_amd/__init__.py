@@ -371,7 +371,8 @@ class FFT2D:
 
 
 class RealFFT2D:
-    """genfft::RealFFT2D<T>(width, height).forward (FFTReal.h:71-184)."""
+    """genfft::RealFFT2D<T>(width, height): forward, and the inverse the reference leaves as a TODO
+    (FFTReal.h:71-184)."""
 
     def __init__(self, width: int = 0, height: int = 0, dtype: torch.dtype = torch.float32):
         self.width, self.height = int(width), int(height)
@@ -387,6 +388,14 @@ class RealFFT2D:
         dev = _check_gpu(out, inp)
         _plan(C.GFFT_R2C_2D, self.width, self._prec, dev, n2=self.height, in_stride=int(in_stride),
               out_stride=int(out_stride)).execute(inp, out)
+
+    def inverse(self, out, out_stride, inp, in_stride) -> None:
+        """FFTReal.h:120-130 (a TODO in the reference): unscaled inverse of
+        forward.  ``inp``: complex spectrum, columns 0..width/2 read, pitch
+        ``in_stride`` complex; ``out``: real, pitch ``out_stride`` scalars (even)."""
+        dev = _check_gpu(out, inp)
+        _plan(C.GFFT_C2R_2D, self.width, self._prec, dev, n2=self.height, in_stride=int(in_stride),
+              out_stride=int(out_stride)).execute(inp, out, inverse=True)
 
 
 def separate_2x_real_FFT(out1: torch.Tensor, out2: torch.Tensor, inp: torch.Tensor, n: int) -> None:

@@ -28,6 +28,7 @@ GFFT_R2C_2D = 3
 GFFT_COLS = 4
 GFFT_DIT = 5
 GFFT_C2R = 6
+GFFT_C2R_2D = 7
 
 GFFT_F32 = 0
 GFFT_F64 = 1

@@ -767,6 +767,53 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         }
         break;
     }
+    case GFFT_C2R_2D: {
+        // inverse of R2C_2D: the n/2+1 non-redundant columns of the spectrum
+        // (height N, input pitch in complex) are inverse-transformed into
+        // scratch region B; each row is then a half spectrum, turned into M
+        // reals by the c2r (pack + inverse c2c) into the output (pitch in scalars)
+        const int64_t M = n, N = d.n2, vcols = M / 2 + 1, L = M / 2;
+        const int64_t ip = dflt(d.in_stride, M), op = dflt(d.out_stride, M);
+        const int64_t id = dflt(d.in_dist, N * ip), od = dflt(d.out_dist, N * op);
+        p->in_bytes = ((batch - 1) * id + (N - 1) * ip + vcols) * cs;
+        p->out_bytes = ((batch - 1) * od + (N - 1) * op + M) * rs;
+        p->out_has_gaps = op > M || od > N * op;
+        p->dir_mask = 2;
+        if (M < 2) return fail(GFFT_ERR_UNSUPPORTED, "C2R_2D needs width >= 2");
+        if ((op % 2) || (od % 2)) return fail(GFFT_ERR_UNSUPPORTED, "C2R_2D output pitch must be even");
+        const int64_t bsz = N * vcols;  // region B: batch x N x vcols, row pitch vcols
+        const size_t c0 = p->launches.size();
+        st = b.columns(N, vcols, batch, B_IN, ip, id, B_SCR, vcols, bsz, true, true);
+        const size_t c1 = p->launches.size();
+        std::vector<size_t> packs;
+        const bool uniform = od == N * op;
+        for (int64_t m = 0; m < (uniform ? 1 : batch) && !st; m++) {
+            const int64_t rowsn = uniform ? N * batch : N;
+            st = b.unpack(M, rowsn, B_SCR, vcols, B_OUT, op / 2, false, true);
+            if (st) break;
+            packs.push_back(p->launches.size() - 1);
+            p->launches.back().in_off = m * bsz * (int64_t)cs;
+            p->launches.back().out_off = m * (od / 2) * (int64_t)cs;
+            st = b.c2c_1d(L, rowsn, B_OUT, m * (od / 2), op / 2, B_OUT, m * (od / 2), op / 2, true, true);
+        }
+        if (st) break;
+        // region B goes after everything else the plan keeps in scratch (the
+        // column sub-pass intermediate, the four-step ring), which are only
+        // live before / after it is written and read
+        const int64_t ring = b.scratch_elems;
+        // the launches that write region B: the single column pass or the
+        // second sub-pass (the last launch), or every copy of a height-1 plane
+        for (size_t i = c0; i < c1; i++) {
+            Launch &l = p->launches[i];
+            if (i + 1 == c1 || l.type == LT_COPY) {
+                if (l.out_buf != B_SCR) return fail(GFFT_ERR_UNSUPPORTED, "internal: C2R_2D column output");
+                l.out_off += ring * (int64_t)cs;
+            }
+        }
+        for (size_t i : packs) p->launches[i].in_off += ring * (int64_t)cs;
+        b.scratch_elems = ring + bsz * batch;
+        break;
+    }
     default:
         return fail(GFFT_ERR_INVALID_ARG, "unknown transform kind %d", d.kind);
     }
@@ -1435,11 +1482,11 @@ static int check_desc(const gfft_desc &d)
 {
     if (d.precision != GFFT_F32 && d.precision != GFFT_F64 && d.precision != GFFT_F16 && d.precision != GFFT_BF16)
         return fail(GFFT_ERR_INVALID_ARG, "precision must be GFFT_F32, GFFT_F64, GFFT_F16 or GFFT_BF16");
-    if (d.kind < GFFT_C2C || d.kind > GFFT_C2R) return fail(GFFT_ERR_INVALID_ARG, "unknown kind %d", d.kind);
+    if (d.kind < GFFT_C2C || d.kind > GFFT_C2R_2D) return fail(GFFT_ERR_INVALID_ARG, "unknown kind %d", d.kind);
     if (d.batch < 1) return fail(GFFT_ERR_INVALID_ARG, "batch must be >= 1");
     if (!is_pow2(d.n) || d.n > ((int64_t)1 << 23))
         return fail(GFFT_ERR_UNSUPPORTED, "size %lld is not a power of two in 1..2^23", (long long)d.n);
-    const bool two = d.kind == GFFT_C2C_2D || d.kind == GFFT_R2C_2D || d.kind == GFFT_COLS;
+    const bool two = d.kind == GFFT_C2C_2D || d.kind == GFFT_R2C_2D || d.kind == GFFT_COLS || d.kind == GFFT_C2R_2D;
     if (two && d.kind != GFFT_COLS && (!is_pow2(d.n2) || d.n2 > ((int64_t)1 << 23)))
         return fail(GFFT_ERR_UNSUPPORTED, "height %lld is not a power of two", (long long)d.n2);
     if (d.kind == GFFT_COLS && d.n2 < 1) return fail(GFFT_ERR_INVALID_ARG, "column count must be >= 1");
@@ -1447,7 +1494,8 @@ static int check_desc(const gfft_desc &d)
         return fail(GFFT_ERR_INVALID_ARG, "negative stride or distance");
     if (d.reserved != 0) return fail(GFFT_ERR_INVALID_ARG, "reserved field must be 0");
     const int64_t width = d.kind == GFFT_COLS ? d.n2 : d.n;
-    if (two && ((d.in_stride && d.in_stride < width) || (d.out_stride && d.out_stride < width)))
+    const int64_t in_width = d.kind == GFFT_C2R_2D ? d.n / 2 + 1 : width;  // C2R_2D reads n/2+1 columns
+    if (two && ((d.in_stride && d.in_stride < in_width) || (d.out_stride && d.out_stride < width)))
         return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the row length");
     if (d.precision == GFFT_F16 || d.precision == GFFT_BF16) {
         if (d.kind != GFFT_C2C && d.kind != GFFT_C2C_2D && d.kind != GFFT_COLS)

@@ -28,6 +28,9 @@
  *        -> GFFT_C2C_2D plan
  *   RealFFT2D<T>::forward             include/genFFT/FFTReal.h:83-104
  *        -> GFFT_R2C_2D plan
+ *   RealFFT2D<T>::inverse             include/genFFT/FFTReal.h:120-130 (declared,
+ *                                     body is assert(!"TODO") in the reference)
+ *        -> GFFT_C2R_2D plan
  *   backend::GetImpl / GetVertImpl / GetDITImpl (the plan factory seam,
  *   include/genFFT/FFTBackend.h:40-70) -> the plan cache of the C++ layer.
  *
@@ -67,6 +70,10 @@ extern "C" {
 #define GFFT_COLS 4     /* column FFTs (FFTVert): n = column length, n2 = columns */
 #define GFFT_DIT 5      /* real-FFT unpack only (DIT::apply), n = real length     */
 #define GFFT_C2R 6      /* batched half-spectrum -> real, unscaled inverse         */
+#define GFFT_C2R_2D 7   /* 2D inverse of GFFT_R2C_2D (RealFFT2D::inverse): n = width */
+                        /* (reals per row), n2 = height; reads columns 0..n/2 of   */
+                        /* the complex input (pitch in complex), writes reals      */
+                        /* (pitch in scalars, even); unscaled                      */
 
 /* precisions */
 #define GFFT_F32 0      /* complex64  (interleaved float)  */
@@ -91,7 +98,7 @@ extern "C" {
 typedef struct gfft_plan_s *gfft_plan;
 
 typedef struct gfft_desc {
-    int32_t kind;       /* GFFT_C2C ... GFFT_C2R                                   */
+    int32_t kind;       /* GFFT_C2C ... GFFT_C2R_2D                                */
     int32_t precision;  /* GFFT_F32 / GFFT_F64 / GFFT_F16 / GFFT_BF16                */
     int64_t n;          /* 1D length; 2D width (columns); COLS column length (rows) */
     int64_t n2;         /* 2D height (rows); COLS number of columns; else ignored   */
@@ -115,7 +122,8 @@ GFFT_API int gfft_plan_destroy(gfft_plan plan);
 
 /* Executes on device-resident buffers, asynchronously on `stream`
  * (a hipStream_t; NULL = the null stream).  in == out is allowed for every
- * kind except GFFT_COLS / GFFT_C2C_2D / GFFT_R2C_2D with different pitches.
+ * kind except GFFT_COLS / GFFT_C2C_2D / GFFT_R2C_2D / GFFT_C2R_2D with different
+ * pitches (C2R_2D also needs out != in: its output is real).
  * direction: GFFT_FORWARD or GFFT_INVERSE (R2C / R2C_2D / DIT are forward
  * only, C2R inverse only).  Calls that share a plan may not overlap in time
  * (they share the plan's workspace). */

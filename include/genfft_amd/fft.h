@@ -347,6 +347,16 @@ public:
         FFT2D<T>(w, h).template transform<false>(out, out_stride, out, out_stride);
     }
 
+    /// FFTReal.h:120-130 (the reference's body is assert(!"TODO")): the
+    /// unscaled inverse of forward -- reads columns 0..width/2 of the complex
+    /// spectrum `in` (pitch in complex elements), writes width x height reals
+    /// (pitch in scalars, even); inverse(forward(x)) = width * height * x.
+    void inverse(T *out, int out_stride, const std::complex<T> *in, int in_stride)
+    {
+        auto p = detail::get_plan(GFFT_C2R_2D, detail::precision<T>(), w, h, 1, in_stride, out_stride, 0, 0, 0);
+        detail::run(p, in, out, true);
+    }
+
     int cols() const { return w; }
     int rows() const { return h; }
 

@@ -238,6 +238,20 @@ void TestRealFFT2D(int w, int h)
     std::vector<std::complex<T>> ref((size_t)h * w);
     cfft.template transform<false>(ref.data(), w, cin.data(), w);
     EXPECT(rel_l2(out.data(), ref.data(), w * h) <= 2 * RelTol<T>(w * h), "RealFFT2D %dx%d", w, h);
+    // inverse (a TODO in the reference, FFTReal.h:120-130): unscaled, output pitch w + 2 scalars
+    std::vector<T> back((size_t)h * (w + 2), T(-3));
+    fft.inverse(back.data(), w + 2, out.data(), w);
+    double num = 0, den = 0;
+    bool gap_ok = true;
+    for (int r = 0; r < h; r++) {
+        for (int c = 0; c < w; c++) {
+            const double d = (double)back[(size_t)r * (w + 2) + c] / ((double)w * h) - (double)in[(size_t)r * w + c];
+            num += d * d;
+            den += (double)in[(size_t)r * w + c] * in[(size_t)r * w + c];
+        }
+        gap_ok = gap_ok && back[(size_t)r * (w + 2) + w] == T(-3) && back[(size_t)r * (w + 2) + w + 1] == T(-3);
+    }
+    EXPECT(std::sqrt(num / den) <= 2 * RelTol<T>(w * h) && gap_ok, "RealFFT2D::inverse %dx%d", w, h);
     g_pass++;
 }
 

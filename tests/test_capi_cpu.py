@@ -204,3 +204,24 @@ def test_16bit_storage_rejections():
     assert explain(desc(prec=C.GFFT_BF16, n=1))[0] == C.GFFT_ERR_UNSUPPORTED
     assert explain(desc(prec=4, n=64))[0] == C.GFFT_ERR_INVALID_ARG
     assert lib.gfft_execute_scaled(None, None, None, 0, None, 0, None) == C.GFFT_ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
+@pytest.mark.parametrize("w,h", [(2, 1), (4, 1), (2, 8), (32, 16), (8, 64), (1024, 512), (4096, 4096),
+                                 (8192, 8192), (16, 4096), (4096, 2), (32768, 4)])
+def test_c2r_2d_plans_in_bounds(prec, w, h):
+    """RealFFT2D::inverse (FFTReal.h:120-130): column inverse into scratch
+    region B, then row c2r into the real output; dense, padded and batched."""
+    for ins, outs, b in ((0, 0, 1), (w // 2 + 1, w + 2, 1), (w + 3, w + 6, 1), (0, 0, 3)):
+        st, text = explain(desc(kind=C.GFFT_C2R_2D, prec=prec, n=w, n2=h, ins=ins, outs=outs, batch=b))
+        if (prec == C.GFFT_F64 and w // 2 > 4096 * 2) or w // 2 > (1 << 22):
+            continue
+        assert st == C.GFFT_OK, (w, h, ins, outs, b, C.lib().gfft_last_error())
+        assert "pack" in text
+
+
+def test_c2r_2d_rejections():
+    assert explain(desc(kind=C.GFFT_C2R_2D, n=1, n2=4))[0] == C.GFFT_ERR_UNSUPPORTED
+    assert explain(desc(kind=C.GFFT_C2R_2D, n=16, n2=4, outs=17))[0] == C.GFFT_ERR_UNSUPPORTED
+    assert explain(desc(kind=C.GFFT_C2R_2D, n=16, n2=4, ins=8))[0] == C.GFFT_ERR_INVALID_ARG
+    assert explain(desc(kind=8, n=16, n2=4))[0] == C.GFFT_ERR_INVALID_ARG

@@ -191,3 +191,88 @@ def test_fft2d_8192_round_trip(gpu):
     err = (torch.linalg.vector_norm((z / (h * w) - x).to(torch.complex128)) /
            torch.linalg.vector_norm(x.to(torch.complex128))).item()
     assert err <= 2 * O.tol_c64(h * w)
+
+
+# ---------------------------------------------------------------------------
+# RealFFT2D<T>::inverse (FFTReal.h:120-130; the reference body is
+# assert(!"TODO")): the unscaled inverse of forward, GFFT_C2R_2D.
+# ---------------------------------------------------------------------------
+def _ifft2_ld_real(X: np.ndarray) -> np.ndarray:
+    """Re(unscaled inverse 2D DFT) of a full spectrum [..., h, w], long double."""
+    rows = O.fft_ld(X, inv=True)
+    cols = np.swapaxes(O.fft_ld(np.ascontiguousarray(np.swapaxes(rows, -1, -2)), inv=True), -1, -2)
+    return cols.real
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("w,h", [(2, 1), (4, 1), (2, 8), (8, 4), (32, 16), (256, 64), (64, 4096), (1024, 256)])
+def test_real2d_inverse_vs_oracle(gpu, prec, w, h):
+    """inverse of a Hermitian spectrum (the forward's output) against the
+    oracle's inverse 2D DFT (real part), and inverse(forward(x)) = w*h*x."""
+    import genfft_amd as G
+    rdt = np.float32 if prec == "f32" else np.float64
+    ndt, tdt = _cdt(prec)
+    rt = torch.float32 if prec == "f32" else torch.float64
+    rng = np.random.default_rng(3 * w + h)
+    x = rng.standard_normal((h, w)).astype(rdt)
+    f = G.RealFFT2D(w, h, rt)
+    spec = torch.empty(h * w, dtype=tdt, device=gpu)
+    f.forward(spec, w, torch.from_numpy(x).to(gpu), w)
+    back = torch.full((h * w,), 7.0, dtype=rt, device=gpu)
+    f.inverse(back, w, spec, w)
+    y = back.cpu().numpy().reshape(h, w)
+    ref = _ifft2_ld_real(spec.cpu().numpy().reshape(h, w).astype(np.complex128))
+    assert O.rel_l2(y, ref) <= _tol(prec, w * h)
+    assert O.rel_l2(y / (w * h), x) <= 2 * _tol(prec, w * h)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_real2d_inverse_padded_and_batched(gpu, prec):
+    """Padded pitches (input pitch w/2+1 and w+3 complex, output pitch w+2
+    scalars), pitch gaps untouched, and three planes in one plan."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rdt = np.float32 if prec == "f32" else np.float64
+    ndt, tdt = _cdt(prec)
+    rt = torch.float32 if prec == "f32" else torch.float64
+    w, h = 64, 32
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((h, w)).astype(rdt)
+    spec = torch.empty(h * w, dtype=tdt, device=gpu)
+    G.RealFFT2D(w, h, rt).forward(spec, w, torch.from_numpy(x).to(gpu), w)
+    S = spec.view(h, w)
+    for ins in (w // 2 + 1, w + 3):
+        inp = torch.full((h, ins), 99 + 99j, dtype=tdt, device=gpu)
+        inp[:, :w // 2 + 1] = S[:, :w // 2 + 1]     # only columns 0..w/2 are read
+        outs = w + 2
+        out = torch.full((h, outs), -5.0, dtype=rt, device=gpu)
+        G.RealFFT2D(w, h, rt).inverse(out, outs, inp, ins)
+        o = out.cpu().numpy()
+        assert O.rel_l2(o[:, :w] / (w * h), x) <= 2 * _tol(prec, w * h)
+        assert np.all(o[:, w:] == -5.0), "write into the output pitch gap"
+    # three planes, dense, one plan
+    xs = rng.standard_normal((3, h, w)).astype(rdt)
+    specs = torch.stack([torch.empty(h * w, dtype=tdt, device=gpu) for _ in range(3)])
+    for i in range(3):
+        G.RealFFT2D(w, h, rt).forward(specs[i], w, torch.from_numpy(xs[i]).to(gpu), w)
+    outb = torch.empty(3, h, w, dtype=rt, device=gpu)
+    p = G.Plan(C.GFFT_C2R_2D, w, precision=0 if prec == "f32" else 1, n2=h, batch=3)
+    p.execute(specs, outb, inverse=True)
+    assert O.rel_l2(outb.cpu().numpy() / (w * h), xs) <= 2 * _tol(prec, w * h)
+
+
+@pytest.mark.parametrize("prec,n", [("f32", 4096), ("f32", 8192), ("f64", 4096)])
+def test_real2d_inverse_large_round_trip(gpu, prec, n):
+    """n x n real planes (the 2D configs): inverse(forward(x)) = n^2 x."""
+    import genfft_amd as G
+    ndt, tdt = _cdt(prec)
+    rt = torch.float32 if prec == "f32" else torch.float64
+    g = torch.Generator(device=gpu).manual_seed(n)
+    x = torch.randn(n, n, device=gpu, generator=g, dtype=rt)
+    spec = torch.empty(n * n, dtype=tdt, device=gpu)
+    f = G.RealFFT2D(n, n, rt)
+    f.forward(spec, n, x, n)
+    back = torch.empty(n, n, device=gpu, dtype=rt)
+    f.inverse(back, n, spec, n)
+    err = (torch.linalg.vector_norm((back / (n * n) - x).double()) / torch.linalg.vector_norm(x.double())).item()
+    assert err <= 2 * _tol(prec, n * n)
