@@ -27,6 +27,7 @@ The JSON line also carries
                    rel-L2 / max-abs against the GPU complex128 plan.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-extras]
+                      [--leg fused_filter|uneven_batch|precision]
 """
 import argparse
 import json
@@ -64,6 +65,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the fused-filter, uneven-batch and precision legs")
+    ap.add_argument("--leg", choices=["fused_filter", "uneven_batch", "precision"],
+                    help="run only this leg and print its JSON object (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (scale reference for PMC byte counters)")
     return ap.parse_args()
@@ -250,6 +253,10 @@ def main():
     import genfft_amd as G
     from genfft_amd import _capi as C
 
+    if a.leg:
+        leg = {"fused_filter": bench_filter, "uneven_batch": bench_uneven, "precision": bench_precision}[a.leg]
+        print(json.dumps({a.leg: leg(torch.device("cuda", 0))}), flush=True)
+        return
     prec, n, batch = WORKLOADS[a.workload]
     half = prec in ("f16", "bf16")
     cdt = torch.complex128 if prec == "c128" else torch.complex64

@@ -124,6 +124,42 @@ __device__ __forceinline__ void dft16(cx<T> *v)
     }
 }
 
+// 32-point DFT: n = 2*n1 + n2, two 16-point DFTs over n1, the twiddle
+// W32^(n2*k1), then radix-2 across n2 (k = k1 + 16*k2).
+template <typename T>
+__device__ __forceinline__ void dft32(cx<T> *v)
+{
+    // cos(pi*k/16), sin(pi*k/16), k = 0..8 (the rest by symmetry)
+    constexpr long double C[9] = {1.0L,
+                                  0.98078528040323044912618223613423903697L,
+                                  0.92387953251128675612818318939678828682L,
+                                  0.83146961230254523707878837761790575673L,
+                                  0.70710678118654752440084436210484903928L,
+                                  0.55557023301960222474283081394853287438L,
+                                  0.38268343236508977172845998403039886676L,
+                                  0.19509032201612826784828486847702224093L,
+                                  0.0L};
+    cx<T> a[16], b[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++) {
+        a[n1] = v[2 * n1];
+        b[n1] = v[2 * n1 + 1];
+    }
+    dft16(a);
+    dft16(b);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) {
+        // W32^k1 = (cos(pi k1/16), -sin(pi k1/16)); sin(pi k/16) = cos(pi (8-k)/16)
+        const T c = k1 <= 8 ? (T)C[k1] : -(T)C[16 - k1];
+        const T sn = k1 <= 8 ? (T)C[8 - k1] : (T)C[k1 - 8];
+        cx<T> t = b[k1];
+        if (k1 == 8) t = mul_mi(t);
+        else if (k1 != 0) t = cmul(t, cx<T>{c, -sn});
+        v[k1] = a[k1] + t;
+        v[k1 + 16] = a[k1] - t;
+    }
+}
+
 template <int R, typename T>
 __device__ __forceinline__ void dft(cx<T> *v)
 {
@@ -131,7 +167,8 @@ __device__ __forceinline__ void dft(cx<T> *v)
     else if constexpr (R == 4) dft4(v);
     else if constexpr (R == 8) dft8(v);
     else if constexpr (R == 16) dft16(v);
-    else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
+    else if constexpr (R == 32) dft32(v);
+    else static_assert(R == 2 || R == 4 || R == 8 || R == 16 || R == 32, "unsupported radix");
 }
 
 // ---------------------------------------------------------------------------
