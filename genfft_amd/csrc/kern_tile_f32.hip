@@ -8,8 +8,8 @@
 // >= 64-byte contiguous row segments.
 // Transposing rows (PAT_ROW -> PAT_COL): the second four-step pass, writing
 // element k of row k1 to k1 + N1*k, M rows wide.
-// Every shape keeps 16 elements per thread, so radix-16 butterflies live
-// entirely in registers.
+// Every default shape keeps 16 elements per thread, so radix-16 butterflies
+// live entirely in registers (the radix-32 alternatives hold 32).
 #include "gfft_registry.h"
 
 namespace gfft {
@@ -69,6 +69,8 @@ const KernelEntry kTable[] = {
     make_variant<T, 1024, 4, 256, S_, R_, R_, 8, 8, 16>(),
     make_variant<T, 1024, 8, 512, S_, R_, R_, 16, 16, 4>(),
     make_variant<T, 1024, 2, 128, S_, R_, R_, 16, 16, 4>(),
+    make_variant<T, 1024, 4, 128, S_, R_, R_, 32, 32>(),   // two radix-32 passes, 32 elements per thread
+    make_variant<T, 1024, 8, 256, S_, R_, R_, 32, 32>(),
     make_variant<T, 2048, 2, 256, S_, R_, R_, 8, 16, 16>(),
     make_variant<T, 2048, 4, 512, S_, R_, R_, 16, 16, 8>(),
     make_variant<T, 4096, 2, 512, S_, R_, R_, 16, 16, 16>(),

@@ -47,3 +47,31 @@ def test_autotune_picks_fastest_and_keeps_parity(gpu, tmp_path, n, batch):
         assert f"L={1 << best['n1_log2']} " in text
     _parity(gpu, n)
     C.check(C.lib().gfft_tuning_clear())
+
+
+@pytest.mark.parametrize("n,batch", [(256, 4096), (512, 2048), (1024, 1024), (2048, 512), (4096, 256),
+                                     (8192, 128), (16384, 64), (1 << 20, 4)])
+def test_every_tuner_candidate_keeps_parity(gpu, tmp_path, n, batch):
+    """Each candidate the tuner can pick (registered radix orders, tile
+    widths, threads per tile, four-step splits, pipelined forms), forced
+    through a tuning file one at a time, matches the oracle."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rep = G.autotune(n, batch, torch.complex64, reps=1)
+    path = str(tmp_path / "one.json")
+    try:
+        for c in rep["candidates"]:
+            with open(path, "w") as f:
+                f.write('{"version": 1, "entries": [\n')
+                f.write(f' {{"precision": 0, "n": {n}, "n1_log2": {c["n1_log2"]}, "col_m": {c["col_m"]}, '
+                        f'"row_m": {c["row_m"]}, "pipe_mask": {c["pipe_mask"]}, "row_variant": {c["row_variant"]}}}\n')
+                f.write(']}\n')
+            C.check(C.lib().gfft_tuning_clear())
+            G.load_tuning(path)
+            with G._cache_lock:
+                G._cache.clear()
+            _parity(gpu, n)
+    finally:
+        C.check(C.lib().gfft_tuning_clear())
+        with G._cache_lock:
+            G._cache.clear()
