@@ -54,6 +54,12 @@ WORKLOADS = {
     "f16_n2^20_b256": ("f16", 1 << 20, 256),
 }
 CSIZE = {"c64": 8, "c128": 16, "f16": 4, "bf16": 4}
+# real-input and 2D workloads of the same configuration set: name -> (kind, n, n2, batch), complex64 / float32
+OTHER = {
+    "r2c_f32_n2^20_b256": ("r2c", 1 << 20, 0, 256),
+    "c2d_c64_4096x4096": ("c2c_2d", 4096, 4096, 1),
+    "c2d_c64_8192x8192": ("c2c_2d", 8192, 8192, 1),
+}
 
 
 def parse():
@@ -61,7 +67,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c64_n2^20_b128", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c64_n2^20_b128", choices=sorted(WORKLOADS) + sorted(OTHER))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the fused-filter, uneven-batch and precision legs")
@@ -257,19 +263,34 @@ def main():
         leg = {"fused_filter": bench_filter, "uneven_batch": bench_uneven, "precision": bench_precision}[a.leg]
         print(json.dumps({a.leg: leg(torch.device("cuda", 0))}), flush=True)
         return
-    prec, n, batch = WORKLOADS[a.workload]
+    kind, n2 = "c2c", 0
+    if a.workload in OTHER:
+        kind, n, n2, batch = OTHER[a.workload]
+        prec = "c64"
+    else:
+        prec, n, batch = WORKLOADS[a.workload]
     half = prec in ("f16", "bf16")
     cdt = torch.complex128 if prec == "c128" else torch.complex64
     csize = CSIZE[prec]
     dev = torch.device("cuda", 0)
     gen = torch.Generator(device=dev).manual_seed(0)
-    # re/im i.i.d. N(0,1) from a fixed seed
-    x = torch.complex(torch.randn(batch, n, device=dev, generator=gen, dtype=cdt.to_real()),
-                      torch.randn(batch, n, device=dev, generator=gen, dtype=cdt.to_real()))
-    if half:
-        x = torch.view_as_real(x).to(torch.float16 if prec == "f16" else torch.bfloat16).contiguous()
-    y = torch.empty_like(x)
-    plan = G.Plan(C.GFFT_C2C, n, precision={"c64": 0, "c128": 1, "f16": 2, "bf16": 3}[prec], batch=batch)
+    if kind == "r2c":
+        # float32 i.i.d. N(0,1) rows -> n/2 + 1 bins each
+        x = torch.randn(batch, n, device=dev, generator=gen)
+        y = torch.empty(batch, n // 2 + 1, dtype=torch.complex64, device=dev)
+        plan = G.Plan(C.GFFT_R2C, n, precision=0, batch=batch, half=True)
+    elif kind == "c2c_2d":
+        x = torch.complex(torch.randn(n2, n, device=dev, generator=gen), torch.randn(n2, n, device=dev, generator=gen))
+        y = torch.empty_like(x)
+        plan = G.Plan(C.GFFT_C2C_2D, n, precision=0, n2=n2, batch=batch)
+    else:
+        # re/im i.i.d. N(0,1) from a fixed seed
+        x = torch.complex(torch.randn(batch, n, device=dev, generator=gen, dtype=cdt.to_real()),
+                          torch.randn(batch, n, device=dev, generator=gen, dtype=cdt.to_real()))
+        if half:
+            x = torch.view_as_real(x).to(torch.float16 if prec == "f16" else torch.bfloat16).contiguous()
+        y = torch.empty_like(x)
+        plan = G.Plan(C.GFFT_C2C, n, precision={"c64": 0, "c128": 1, "f16": 2, "bf16": 3}[prec], batch=batch)
 
     if a.calib_copy:
         # a plain copy of known size: reads and writes exactly 1 GiB
@@ -294,8 +315,16 @@ def main():
     ms_total = e0.elapsed_time(e1)
     ms_step = ms_total / a.steps
 
-    flops = 5.0 * n * math.log2(n) * batch
-    alg_bytes = 2.0 * n * csize * batch
+    if kind == "r2c":
+        flops = 2.5 * n * math.log2(n) * batch
+        alg_bytes = (4.0 * n + 8.0 * (n // 2 + 1)) * batch
+    elif kind == "c2c_2d":
+        N = n * n2
+        flops = 5.0 * N * math.log2(N) * batch
+        alg_bytes = 2.0 * N * csize * batch
+    else:
+        flops = 5.0 * n * math.log2(n) * batch
+        alg_bytes = 2.0 * n * csize * batch
     gflops = flops / (ms_step * 1e-3) / 1e9
 
     # per-launch profile (HIP events around each kernel on the execution stream)
@@ -334,7 +363,7 @@ def main():
             traffic = t.get("bytes_per_launch")
 
     cpu = None
-    if not a.no_cpu and not half:
+    if not a.no_cpu and not half and kind == "c2c":
         cpu = cpu_baseline(x.cpu().numpy(), n, batch, prec, y.cpu().numpy())
     del x, y, plan
     torch.cuda.empty_cache()
@@ -357,9 +386,11 @@ def main():
         "vs_baseline": None,
         "dtype": {"c64": "f32", "c128": "f64", "f16": "f16 storage, f32 math", "bf16": "bf16 storage, f32 math"}[prec],
         "data": "synthetic: re/im i.i.d. N(0,1), torch generator seed 0, resident in HBM",
-        "config": {"workload": f"batched 1D c2c forward, "
-                               f"{ {'c64': 'complex64', 'c128': 'complex128', 'f16': 'FP16 pairs', 'bf16': 'BF16 pairs'}[prec]}, "
-                               f"N=2^{int(math.log2(n))}, batch {batch}",
+        "config": {"workload": (f"batched 1D c2c forward, "
+                                f"{ {'c64': 'complex64', 'c128': 'complex128', 'f16': 'FP16 pairs', 'bf16': 'BF16 pairs'}[prec]}, "
+                                f"N=2^{int(math.log2(n))}, batch {batch}") if kind == "c2c" else
+                               (f"batched real float32 -> n/2+1 bins, N=2^{int(math.log2(n))}, batch {batch}"
+                                if kind == "r2c" else f"2D complex64 forward {n}x{n2}"),
                    "n": n, "batch": batch, "precision": prec, "plan": [k for k in groups]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_key,

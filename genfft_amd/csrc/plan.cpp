@@ -640,8 +640,28 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             break;
         }
         if (id % 2) return fail(GFFT_ERR_UNSUPPORTED, "R2C input distance must be even");
-        st = b.c2c_1d(n / 2, batch, B_IN, 0, id / 2, B_OUT, 0, od, true, true);
-        if (!st) st = b.unpack(n, batch, B_OUT, od, B_OUT, od, half, false);
+        if (od % 2 == 0) {
+            st = b.c2c_1d(n / 2, batch, B_IN, 0, id / 2, B_OUT, 0, od, true, true);
+            if (!st) st = b.unpack(n, batch, B_OUT, od, B_OUT, od, half, false);
+            break;
+        }
+        // An odd output distance (the dense n/2+1 half spectrum) would force
+        // 8-byte stores in every tile pass: the n/2-point transform goes
+        // densely into scratch instead, after the four-step ring, and the
+        // unpack writes the output.
+        {
+            const int64_t L = n / 2;
+            const size_t c0 = p->launches.size();
+            st = b.c2c_1d(L, batch, B_IN, 0, id / 2, B_SCR, 0, L, true, true);
+            if (st) break;
+            const int64_t ring = b.scratch_elems;
+            for (size_t i = c0; i < p->launches.size(); i++)
+                if (p->launches[i].last && p->launches[i].out_buf == B_SCR) p->launches[i].out_off += ring * (int64_t)cs;
+            st = b.unpack(n, batch, B_SCR, L, B_OUT, od, half, false);
+            if (st) break;
+            p->launches.back().in_off = ring * (int64_t)cs;
+            b.scratch_elems = ring + L * batch;
+        }
         break;
     }
     case GFFT_C2R: {
