@@ -46,6 +46,9 @@ const KernelEntry kTable[] = {
     make_entry<T, 1024, 8, 512, S_, R_, C_, 16, 16, 4>(),
     make_entry<T, 2048, 4, 512, S_, R_, C_, 16, 16, 8>(),
     make_entry<T, 4096, 2, 512, S_, R_, C_, 16, 16, 16>(),
+    // --- persistent pipelined forms of the 2^20 four-step passes ---
+    make_pipe_entry<T, 1024, 8, 512, E_, C_, C_, 16, 16, 4>(),
+    make_pipe_entry<T, 1024, 8, 512, S_, R_, C_, 16, 16, 4>(),
     // --- autotuner alternatives ---
     make_variant<T, 1024, 4, 256, S_, R_, R_, 16, 16, 4>(),
     make_variant<T, 1024, 2, 128, S_, R_, R_, 4, 16, 16>(),
