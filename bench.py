@@ -47,7 +47,11 @@ WORKLOADS = {
     "c64_n2^14_b8192": ("c64", 1 << 14, 8192),
     "c64_n2^18_b512": ("c64", 1 << 18, 512),
     "c64_n2^22_b32": ("c64", 1 << 22, 32),
+    "c128_n2^10_b65536": ("c128", 1 << 10, 65536),
+    "c128_n2^14_b4096": ("c128", 1 << 14, 4096),
+    "c128_n2^18_b256": ("c128", 1 << 18, 256),
     "c128_n2^20_b64": ("c128", 1 << 20, 64),
+    "c128_n2^22_b16": ("c128", 1 << 22, 16),
     # 16-bit storage (FP32 arithmetic), 4 bytes per complex element
     "f16_n2^20_b64": ("f16", 1 << 20, 64),
     "bf16_n2^20_b64": ("bf16", 1 << 20, 64),

@@ -289,11 +289,13 @@ void four_step_split(int prec, int lg, int *l1, int *l2, int n1_override = 0)
 {
     int b = lg / 2, a = lg - b;  // keep the column length <= the row length
     if (a > b) std::swap(a, b);
-    // complex64: the transposing row pass runs at copy speed at length 1024
-    // (pipelined kernel) while column passes are fastest short, so keep the
-    // row length at 1024 while the column length fits a tile (measured on
-    // MI355X for 2 GiB batches: 2^16 0.79 ms, 2^18 0.81 ms, 2^20 0.92 ms)
-    if (prec == 0 && lg - 10 >= 1 && ((int64_t)1 << (lg - 10)) <= max_len(prec, PAT_COL, PAT_COL)) {
+    // The transposing row pass runs at copy speed at length 1024 (pipelined
+    // kernel) while column passes are fastest short, so keep the row length
+    // at 1024 while the column length fits a tile.  Measured on MI355X for
+    // 2 GiB complex64 batches: 2^16 0.79 ms, 2^18 0.81 ms, 2^20 0.92 ms; for
+    // 1 GiB complex128 batches: 2^14 1.04 -> 0.81 ms, 2^18 1.16 -> 0.86 ms
+    // against the even split.
+    if (lg - 10 >= 1 && ((int64_t)1 << (lg - 10)) <= max_len(prec, PAT_COL, PAT_COL)) {
         b = 10;
         a = lg - 10;
     }
