@@ -904,4 +904,165 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent pipelined form over 16-bit storage: the same loop as
+// tile_fft_pipe_kernel with the next tile prefetched as packed 32-bit words
+// (half the registers of complex64) and the per-transform scale of
+// tile_fft_kernel_h.
+// ---------------------------------------------------------------------------
+template <class S, int L, int M, int NT, class G, int INP>
+__device__ __forceinline__ bool fetch_tile_h(uint32_t (&reg)[M * L / NT], const TileArgs &a, int o2, int o, int s0,
+                                             int nvalid, int tid)
+{
+    constexpr int TOT = M * L;
+    const uint32_t *in = reinterpret_cast<const uint32_t *>(a.in) + in_base(a, o2, o);
+    if constexpr (INP == PAT_ROW) in += (int64_t)s0 * a.in_inner;
+    else in += s0;
+    if constexpr (TOT % (4 * NT) == 0 && (INP == PAT_ROW ? L : M) % 4 == 0) {
+        if (a.vec_in && nvalid == M) {
+#pragma unroll
+            for (int it = 0; it < TOT / (4 * NT); it++) {
+                const int e = (tid + it * NT) * 4;
+                const uint4 *p;
+                if constexpr (INP == PAT_ROW) p = reinterpret_cast<const uint4 *>(in + (int64_t)(e / L) * a.in_inner + e % L);
+                else p = reinterpret_cast<const uint4 *>(in + (int64_t)(e / M) * a.in_elem + e % M);
+                const uint4 q = *p;
+                reg[4 * it] = q.x;
+                reg[4 * it + 1] = q.y;
+                reg[4 * it + 2] = q.z;
+                reg[4 * it + 3] = q.w;
+            }
+            return true;
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < TOT / NT; it++) {
+        const int e = tid + it * NT;
+        const int s = INP == PAT_ROW ? e / L : e % M;
+        const int n = INP == PAT_ROW ? e % L : e / M;
+        if (s < nvalid) reg[it] = INP == PAT_ROW ? in[(int64_t)s * a.in_inner + n] : in[(int64_t)n * a.in_elem + s];
+    }
+    return false;
+}
+
+template <class S, int L, int M, int NT, class G, int INP>
+__device__ __forceinline__ void put_tile_h(cx<float> *lds, const uint32_t (&reg)[M * L / NT], bool vec,
+                                           const TileArgs &a, int o, int nvalid, int tid)
+{
+    constexpr int TOT = M * L;
+    const bool sw = swap_in_of(a, o);
+    if constexpr (TOT % (4 * NT) == 0 && (INP == PAT_ROW ? L : M) % 4 == 0) {
+        if (vec) {
+#pragma unroll
+            for (int it = 0; it < TOT / (4 * NT); it++) {
+                const int e = (tid + it * NT) * 4;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const cx<float> x = hdec<S>(reg[4 * it + v]);
+                    if constexpr (INP == PAT_ROW) lds[G::off(e / L, e % L + v)] = sw ? swap_ri(x) : x;
+                    else lds[G::off(e % M + v, e / M)] = sw ? swap_ri(x) : x;
+                }
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < TOT / NT; it++) {
+        const int e = tid + it * NT;
+        const int s = INP == PAT_ROW ? e / L : e % M;
+        const int n = INP == PAT_ROW ? e % L : e / M;
+        if (s < nvalid) {
+            const cx<float> x = hdec<S>(reg[it]);
+            lds[G::off(s, n)] = sw ? swap_ri(x) : x;
+        }
+    }
+}
+
+template <class S, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
+__global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
+{
+    using T = float;
+    using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
+    constexpr int TWN = pass_tw_count<1, RS...>();
+    constexpr int E = M * L / NT;
+    constexpr int LAST_R = last_radix<RS...>();
+    __shared__ cx<T> lds[G::ELEMS];
+    __shared__ cx<T> twl[TWN > 0 ? TWN : 1];
+    __shared__ float scl[M];
+    const int tid = threadIdx.x;
+    {
+        const cx<T> *gtw = reinterpret_cast<const cx<T> *>(a.pass_tw);
+        for (int i = tid; i < TWN; i += NT) twl[i] = gtw[i];
+    }
+    const int grid = gridDim.x;
+    const int slot = a.xcd_remap ? xcd_tile(blockIdx.x, grid) : (int)blockIdx.x;
+    auto decode = [&](int t, int &o2, int &o, int &s0, int &nv) {
+        const int t2 = t / a.tiles_inner;
+        const int ti = t - t2 * a.tiles_inner;
+        o2 = t2 / a.n_outer1;
+        o = t2 - o2 * a.n_outer1;
+        s0 = ti * M;
+        nv = min(M, a.n_inner - s0);
+    };
+    uint32_t reg[E];
+    int tile = slot, o2 = 0, o = 0, s0 = 0, nvalid = 0;
+    bool vec = false;
+    if (tile < a.n_tiles) {
+        decode(tile, o2, o, s0, nvalid);
+        vec = fetch_tile_h<S, L, M, NT, G, INP>(reg, a, o2, o, s0, nvalid, tid);
+    }
+    while (tile < a.n_tiles) {
+        // thread index re-materialised per iteration (see tile_fft_pipe_kernel)
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        cx<double> hj{}, lj{}, hd{}, ld{}, hr{}, lr{};
+        const bool tw = LAYOUT == LAY_ELEM && NT % M == 0 && a.tw_mode != TW_NONE;
+        if (tw) {
+            const uint32_t c = a.tw_mode == TW_INNER ? (uint32_t)(s0 + t % M) : (uint32_t)o;
+            const cx<double> *lo = reinterpret_cast<const cx<double> *>(a.tw_lo);
+            const cx<double> *hi = reinterpret_cast<const cx<double> *>(a.tw_hi);
+            const uint32_t lm = (1u << a.tw_lo_bits) - 1u;
+            const uint32_t mj = (c * (uint32_t)(t / M)) & a.tw_mask;
+            const uint32_t md = (c * (uint32_t)(NT / M)) & a.tw_mask;
+            const uint32_t mr = (c * (uint32_t)(L / LAST_R)) & a.tw_mask;
+            hj = hi[mj >> a.tw_lo_bits];
+            lj = lo[mj & lm];
+            hd = hi[md >> a.tw_lo_bits];
+            ld = lo[md & lm];
+            hr = hi[mr >> a.tw_lo_bits];
+            lr = lo[mr & lm];
+        }
+        put_tile_h<S, L, M, NT, G, INP>(lds, reg, vec, a, o, nvalid, t);
+        if (a.scale) {
+            const int64_t base = in_base(a, o2, o);
+            for (int s = t; s < nvalid; s += NT) {
+                const int64_t q = base + (INP == PAT_ROW ? (int64_t)(s0 + s) * a.in_inner : (int64_t)(s0 + s));
+                scl[s] = a.scale[(a.scale_off + q) / a.scale_div];
+            }
+        }
+        Tw4Pre pre{};
+        if (tw) {
+            pre.hj = cmul(hj, lj);
+            pre.hd = cmul(hd, ld);
+            pre.hr = cmul(hr, lr);
+        }
+        __syncthreads();
+        const int next = tile + grid;
+        int no2 = 0, no = 0, ns0 = 0, nnv = 0;
+        if (next < a.n_tiles) {
+            decode(next, no2, no, ns0, nnv);
+            vec = fetch_tile_h<S, L, M, NT, G, INP>(reg, a, no2, no, ns0, nnv, t);
+        }
+        stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, (LAYOUT == LAY_ELEM && NT % M == 0), 1, RS...>(
+            lds, twl, t, a, s0, o, pre);
+        store_tile_h<S, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, t, a.scale ? scl : nullptr);
+        __syncthreads();
+        tile = next;
+        o2 = no2;
+        o = no;
+        s0 = ns0;
+        nvalid = nnv;
+    }
+}
+
 }  // namespace gfft

@@ -86,6 +86,17 @@ constexpr KernelEntry make_entry_h()
     return e;
 }
 
+// Pipelined 16-bit storage form (tile_fft_pipe_kernel_h).
+template <class S, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
+constexpr KernelEntry make_pipe_entry_h()
+{
+    KernelEntry e = make_pipe_entry<float, L, M, NT, LAY, INP, OUTP, RS...>();
+    e.lds_bytes += M * (int)sizeof(float);
+    e.store = std::is_same<S, st_f16>::value ? 1 : 2;
+    e.fn = &tile_fft_pipe_kernel_h<S, L, M, NT, LAY, INP, OUTP, RS...>;
+    return e;
+}
+
 // Defined in kern_tile_f32.hip / kern_tile_f64.hip / kern_tile_h.hip.
 const KernelEntry *tile_kernels_f32(int *count);
 const KernelEntry *tile_kernels_f64(int *count);

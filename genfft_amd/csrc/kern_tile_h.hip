@@ -49,6 +49,9 @@ struct HalfTable {
         make_entry_h<S, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
         make_entry_h<S, 2048, 8, 1024, S_, R_, C_, 16, 16, 8>(),
         make_entry_h<S, 4096, 4, 1024, S_, R_, C_, 16, 16, 16>(),
+        // --- persistent pipelined forms of the 2^20 four-step passes ---
+        make_pipe_entry_h<S, 1024, 16, 1024, E_, C_, C_, 8, 8, 16>(),
+        make_pipe_entry_h<S, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
     };
 };
 }  // namespace
