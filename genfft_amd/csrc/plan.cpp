@@ -175,6 +175,46 @@ int get_pass_twiddles(int dev, const KernelEntry &k, const void **out, bool dry)
     return GFFT_OK;
 }
 
+// W_N^m for m <= N/2 as one FP64 table (N not a power of two): the same
+// TwTable form with a mask of all ones and an empty high level, so
+// hi[m >> lo_bits] = hi[0] = 1 exactly.
+int get_direct(int dev, int64_t N, TwTable *t, bool dry)
+{
+    auto &T = tables();
+    std::lock_guard<std::mutex> lk(T.mu);
+    t->mask = ~0ull;
+    t->lo_bits = 62;
+    t->pad = 0;
+    if (dry) {
+        t->lo = t->hi = nullptr;
+        return GFFT_OK;
+    }
+    auto key = std::make_pair(dev, -N);  // negative key: the direct tables
+    auto it = T.two_level.find(key);
+    if (it != T.two_level.end()) {
+        t->lo = (const double *)it->second.first;
+        t->hi = (const double *)it->second.second;
+        return GFFT_OK;
+    }
+    const int64_t cnt = N / 2 + 1;
+    std::vector<double> lo(2 * cnt), hi = {1.0, 0.0};
+    for (int64_t m = 0; m < cnt; m++) {
+        long double c, s;
+        twiddle_ld(m, N, &c, &s);
+        lo[2 * m] = (double)c;
+        lo[2 * m + 1] = (double)s;
+    }
+    void *dlo = nullptr, *dhi = nullptr;
+    int st = upload(&dlo, lo.data(), lo.size() * sizeof(double));
+    if (st) return st;
+    st = upload(&dhi, hi.data(), hi.size() * sizeof(double));
+    if (st) return st;
+    T.two_level[key] = {dlo, dhi};
+    t->lo = (const double *)dlo;
+    t->hi = (const double *)dhi;
+    return GFFT_OK;
+}
+
 int get_two_level(int dev, int64_t N, TwTable *t, bool dry)
 {
     auto &T = tables();
@@ -594,7 +634,7 @@ struct Builder {
         l.ua.out_dist = out_dist;
         l.ua.half = half ? 1 : 0;
         if (n >= 2) {
-            int st = get_two_level(p->device, n, &l.ua.tw, dry);
+            int st = is_pow2(n) ? get_two_level(p->device, n, &l.ua.tw, dry) : get_direct(p->device, n, &l.ua.tw, dry);
             if (st) return st;
         }
         l.in_buf = in_buf;
@@ -1506,8 +1546,12 @@ static int check_desc(const gfft_desc &d)
         return fail(GFFT_ERR_INVALID_ARG, "precision must be GFFT_F32, GFFT_F64, GFFT_F16 or GFFT_BF16");
     if (d.kind < GFFT_C2C || d.kind > GFFT_C2R_2D) return fail(GFFT_ERR_INVALID_ARG, "unknown kind %d", d.kind);
     if (d.batch < 1) return fail(GFFT_ERR_INVALID_ARG, "batch must be >= 1");
-    if (!is_pow2(d.n) || d.n > ((int64_t)1 << 23))
-        return fail(GFFT_ERR_UNSUPPORTED, "size %lld is not a power of two in 1..2^23", (long long)d.n);
+    // DIT (the real-FFT unpack alone) also takes any n divisible by 4, as
+    // genFFT's dynamic FFTDIT<-1> does (generic/fft_dit_impl_generic.inl:117-153)
+    const bool dit_any = d.kind == GFFT_DIT && d.n > 0 && d.n % 4 == 0;
+    if ((!is_pow2(d.n) && !dit_any) || d.n > ((int64_t)1 << 23))
+        return fail(GFFT_ERR_UNSUPPORTED, "size %lld is not a power of two in 1..2^23%s", (long long)d.n,
+                    d.kind == GFFT_DIT ? " or a multiple of 4" : "");
     const bool two = d.kind == GFFT_C2C_2D || d.kind == GFFT_R2C_2D || d.kind == GFFT_COLS || d.kind == GFFT_C2R_2D;
     if (two && d.kind != GFFT_COLS && (!is_pow2(d.n2) || d.n2 > ((int64_t)1 << 23)))
         return fail(GFFT_ERR_UNSUPPORTED, "height %lld is not a power of two", (long long)d.n2);

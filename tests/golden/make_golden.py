@@ -68,6 +68,17 @@ def main():
             y = np.empty(n, dt)
             getattr(r, "ref_transform_interleave_" + sfx)(P(y), P(x), P(x2), n)
             out[f"tinter_{sfx}_n{n}"] = y
+        # DIT<T>(n) for n divisible by 4 but not a power of two: genFFT's
+        # dynamic FFTDIT<-1> (generic/fft_dit_impl_generic.inl:117-153)
+        # (own generator: the inputs of every other entry stay as they were)
+        rng_dit = np.random.default_rng(1212 if sfx == "f32" else 1213)
+        for n in [12, 24, 40, 100, 1000]:
+            z = (rng_dit.uniform(-1, 1, n // 2) + 1j * rng_dit.uniform(-1, 1, n // 2)).astype(dt)
+            out[f"dit_{sfx}_n{n}_in"] = z
+            for half in (0, 1):
+                f = np.full(n, 43 + 21j, dt)
+                getattr(r, "ref_dit_apply_" + sfx)(P(f), P(z), n, half, 1)
+                out[f"dit_{sfx}_n{n}_{'half' if half else 'full'}"] = f
         for n in [8, 64]:
             z = cplx(n, dt)
             out[f"sep_{sfx}_n{n}_in"] = z

@@ -225,3 +225,18 @@ def test_c2r_2d_rejections():
     assert explain(desc(kind=C.GFFT_C2R_2D, n=16, n2=4, outs=17))[0] == C.GFFT_ERR_UNSUPPORTED
     assert explain(desc(kind=C.GFFT_C2R_2D, n=16, n2=4, ins=8))[0] == C.GFFT_ERR_INVALID_ARG
     assert explain(desc(kind=8, n=16, n2=4))[0] == C.GFFT_ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
+def test_dit_plans_for_multiples_of_four(prec):
+    """DIT<T>(n) takes any n divisible by 4, as genFFT's dynamic FFTDIT<-1>
+    (generic/fft_dit_impl_generic.inl:117-153); other kinds stay powers of two."""
+    for n in (12, 20, 100, 1000, 4100, 3 << 18, (1 << 23) - 4):
+        for half in (0, 1):
+            st, text = explain(desc(kind=C.GFFT_DIT, prec=prec, n=n, batch=3, flags=half))
+            assert st == C.GFFT_OK, (n, C.lib().gfft_last_error())
+            assert "unpack" in text
+    for n in (6, 10, 7):
+        assert explain(desc(kind=C.GFFT_DIT, prec=prec, n=n))[0] == C.GFFT_ERR_UNSUPPORTED
+    assert explain(desc(kind=C.GFFT_C2C, prec=prec, n=12))[0] == C.GFFT_ERR_UNSUPPORTED
+    assert explain(desc(kind=C.GFFT_R2C, prec=prec, n=12))[0] == C.GFFT_ERR_UNSUPPORTED

@@ -182,3 +182,41 @@ def test_c2r_ignores_imag_of_dc_and_nyquist(gpu, prec, n):
     full = np.concatenate([Fr, np.conj(Fr[:, 1:n // 2][:, ::-1])], axis=1).astype(np.complex128)
     ref = O.fft_ld(full, inv=True).real
     assert O.rel_l2(a, ref) <= _tol(prec, n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [12, 24, 40, 100, 1000])
+@pytest.mark.parametrize("half", [False, True])
+def test_dit_non_pow2_matches_genfft(gpu, golden, prec, n, half):
+    """DIT<T>(n) for n divisible by 4 but not a power of two against genFFT's
+    dynamic FFTDIT<-1> outputs, with the reference's sentinel past the bins."""
+    import genfft_amd as G
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    z = golden[f"dit_{prec}_n{n}_in"]
+    g = golden[f"dit_{prec}_n{n}_{'half' if half else 'full'}"]
+    nout = n // 2 + 1 if half else n
+    out = torch.full((n,), 43 + 21j, dtype=tcdt, device=gpu)
+    G.DIT(n, tcdt).apply(out, torch.from_numpy(z).to(gpu), half)
+    y = out.cpu().numpy()
+    eps = 1e-12 if prec == "f64" else 1e-5
+    assert np.max(np.abs(y[:nout] - g[:nout])) <= eps * max(1.0, np.max(np.abs(g[:nout])))
+    assert np.all(y[nout:] == 43 + 21j)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [4100, 3 << 18])
+def test_dit_non_pow2_is_the_real_fft(gpu, prec, n):
+    """unpack(FFT_{n/2}(x_even + i x_odd)) = FFT_n(x) for n a multiple of 4,
+    the n/2-point input made by numpy's FP64 FFT, checked against numpy's FP64
+    FFT of the real signal and against the oracle's unpack."""
+    import genfft_amd as G
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, (2, n))
+    z = np.fft.fft(x[:, 0::2] + 1j * x[:, 1::2], axis=-1).astype(cdt)
+    out = torch.empty((2, n), dtype=tcdt, device=gpu)
+    G.DIT(n, tcdt).apply(out, torch.from_numpy(z).to(gpu), False)
+    y = out.cpu().numpy()
+    assert O.rel_l2(y, np.fft.fft(x, axis=-1)) <= 2 * _tol(prec, n)
+    for r in range(2):
+        assert O.rel_l2(y[r], O.unpack(z[r], n, False)) <= _tol(prec, n)

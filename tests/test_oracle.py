@@ -51,9 +51,11 @@ def test_real_forward_matches_genfft(golden, sfx, n, half):
 
 
 @pytest.mark.parametrize("sfx", ["f32", "f64"])
-@pytest.mark.parametrize("n", SIZES_REAL)
+@pytest.mark.parametrize("n", SIZES_REAL + [12, 24, 40, 100, 1000])
 @pytest.mark.parametrize("half", [0, 1])
 def test_unpack_matches_genfft(golden, sfx, n, half):
+    """adjust_DIT_impl, bit-exact against genFFT's generic DIT; the sizes that
+    are not powers of two run genFFT's dynamic FFTDIT<-1>."""
     z = golden[f"dit_{sfx}_n{n}_in"]
     f = O.unpack(z, n, bool(half))
     g = golden[f"dit_{sfx}_n{n}_{'half' if half else 'full'}"]
