@@ -13,6 +13,8 @@ The JSON line also carries
                    `traffic` is the HBM bytes per launch measured by rocprofv3
                    PMC counters, read from profiles/pmc_traffic.json when present;
   e2e           -- the whole transform against the HBM roofline;
+  copy_gbs      -- (in roofline) a plain 1 GiB device copy measured in the same run,
+                   the practical ceiling of the box, with frac_of_copy beside frac;
   cpu_baseline  -- genFFT itself (oracle/_ref, built from the reference sources)
                    timed on the host cores over a bounded sample, or the
                    oracle's C restatement when that library is absent;
@@ -366,6 +368,13 @@ def main():
         if t:
             traffic = t.get("bytes_per_launch")
 
+    # the box's own ceiling: a plain device copy of 1 GiB (torch), best of 5
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    copy_ms = min(_time_ms(lambda: dst.copy_(src), 5, 2) for _ in range(3))
+    copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    del src, dst
+
     cpu = None
     if not a.no_cpu and not half and kind == "c2c":
         cpu = cpu_baseline(x.cpu().numpy(), n, batch, prec, y.cpu().numpy())
@@ -398,9 +407,11 @@ def main():
                    "n": n, "batch": batch, "precision": prec, "plan": [k for k in groups]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_key,
+                     "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
                      "kernel_ms": round(per_launch_ms, 5), "alg_bytes_per_launch": per_launch_bytes},
         "e2e": {"alg_bytes": alg_bytes, "achieved_gbs": round(alg_bytes / (ms_step * 1e-3) / 1e9, 1),
                 "hbm_frac": round(alg_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "frac_of_copy": round(alg_bytes / (ms_step * 1e-3) / 1e9 / copy_gbs, 4),
                 "kernels": {k: {"launches": v["launches"], "ms_per_step": round(v["ms"], 5)}
                             for k, v in groups.items()},
                 "wall_s": round(wall, 3)},
