@@ -1113,6 +1113,17 @@ bool device_ok(int *dev)
     return hipGetDevice(dev) == hipSuccess;
 }
 
+// A plan's tables and workspace live on the device that was current when it
+// was created; its launches must run there.
+int on_plan_device(int plan_device)
+{
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return fail(GFFT_ERR_HIP, "hipGetDevice failed");
+    if (cur != plan_device)
+        return fail(GFFT_ERR_INVALID_ARG, "plan belongs to device %d but device %d is current", plan_device, cur);
+    return GFFT_OK;
+}
+
 
 // ---------------------------------------------------------------------------
 // Uneven batches: transforms of mixed power-of-two lengths at arbitrary
@@ -1636,6 +1647,7 @@ int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void 
     if (!(plan->dir_mask & (1 << direction)))
         return fail(GFFT_ERR_INVALID_ARG, "this plan kind supports only the %s direction",
                     plan->dir_mask == 1 ? "forward" : "inverse");
+    if (int st = on_plan_device(plan->device)) return st;
     return run(plan, in, out, direction, (hipStream_t)stream);
 }
 
@@ -1649,6 +1661,7 @@ int gfft_execute_scaled(gfft_plan plan, const void *in, void *out, int direction
     if (!plan->store) return fail(GFFT_ERR_UNSUPPORTED, "per-transform scales need a GFFT_F16 / GFFT_BF16 plan");
     if (scale_mode != GFFT_SCALE_GIVEN && scale_mode != GFFT_SCALE_AUTO)
         return fail(GFFT_ERR_INVALID_ARG, "scale_mode must be GFFT_SCALE_GIVEN or GFFT_SCALE_AUTO");
+    if (int st = on_plan_device(plan->device)) return st;
     hipStream_t s = (hipStream_t)stream;
     if (scale_mode == GFFT_SCALE_AUTO) {
         AmaxArgs a{};
@@ -1693,6 +1706,7 @@ int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int direction, 
                       int max_launches, int *count)
 {
     if (!plan || !in || !out || !count) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    if (int st = on_plan_device(plan->device)) return st;
     const int n = (int)plan->launches.size();
     *count = n;
     std::vector<hipEvent_t> ev(2 * n);
@@ -1800,6 +1814,7 @@ int gfft_batch_plan_destroy(gfft_batch_plan plan)
 int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode, void *stream)
 {
     if (!plan || !in || !out) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    if (int st = on_plan_device(plan->device)) return st;
     const hipStream_t s = (hipStream_t)stream;
     std::lock_guard<std::mutex> lk(plan->mu);
     if (mode == GFFT_BATCH_EAGER) return enqueue_batch(plan, in, out, s);
@@ -1868,8 +1883,13 @@ struct gfft_filter_plan_s {
     int prec = 0, device = 0;
     int64_t n = 0, batch = 0;
     void *tw = nullptr;        // W_n^k, k < n/2
-    const void *pass_tw = nullptr;
+    const void *pass_tw = nullptr;  // shared device table (not owned)
     void *partial = nullptr;   // batch x (n/2 + 1) complex
+    ~gfft_filter_plan_s()
+    {
+        if (tw) (void)hipFree(tw);
+        if (partial) (void)hipFree(partial);
+    }
 };
 
 extern "C" {
@@ -1917,16 +1937,14 @@ int gfft_filter_plan_create(gfft_filter_plan *plan, int precision, int64_t n, in
 
 int gfft_filter_plan_destroy(gfft_filter_plan plan)
 {
-    if (!plan) return GFFT_OK;
-    if (plan->tw) (void)hipFree(plan->tw);
-    if (plan->partial) (void)hipFree(plan->partial);
-    delete plan;
+    delete plan;  // frees the plan's device buffers
     return GFFT_OK;
 }
 
 int gfft_filter_forward(gfft_filter_plan plan, const void *x, const void *H, void *y, int conj_h, void *stream)
 {
     if (!plan || !x || !H || !y) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    if (int st = on_plan_device(plan->device)) return st;
     FilterArgs a{};
     a.x = x;
     a.H = H;
@@ -1945,6 +1963,7 @@ int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, co
                          void *stream)
 {
     if (!plan || !x || !H || !gy || !gx || !gH) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    if (int st = on_plan_device(plan->device)) return st;
     FilterArgs a{};
     a.x = x;
     a.gy = gy;

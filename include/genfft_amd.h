@@ -114,7 +114,10 @@ typedef struct gfft_desc {
 } gfft_desc;
 
 /* Creates a plan: validates the descriptor, selects kernels, uploads the
- * FP64-generated twiddle tables and allocates the plan's device workspace. */
+ * FP64-generated twiddle tables and allocates the plan's device workspace.
+ * The plan belongs to the HIP device current at creation; every call that
+ * executes it (and the batch and filter plans alike) must be made with that
+ * device current, else it returns GFFT_ERR_INVALID_ARG. */
 GFFT_API int gfft_plan_create(gfft_plan *plan, const gfft_desc *desc);
 
 /* Releases a plan and its workspace. NULL is accepted. */
