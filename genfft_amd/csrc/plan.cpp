@@ -407,6 +407,39 @@ struct Launch {
 }  // namespace
 }  // namespace gfft
 
+// Orders the work of one plan across streams.  A plan's workspace (scratch,
+// partial sums) is shared by all its calls, so a call on a new stream first
+// waits for the plan's previous work; calls on one stream are ordered by the
+// stream itself.  (genFFT's plans are reentrant across threads for distinct
+// buffers, FFTLevel.h:76-92; this keeps the shared-workspace plans so.)
+struct StreamOrder {
+    std::mutex mu;
+    hipEvent_t done = nullptr;
+    hipStream_t last = nullptr;
+    bool used = false;
+    ~StreamOrder()
+    {
+        if (done) (void)hipEventDestroy(done);
+    }
+    // call with mu held, before the plan's launches on s
+    hipError_t before(hipStream_t s)
+    {
+        if (!done) {
+            hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+        }
+        if (used && last != s) return hipStreamWaitEvent(s, done, 0);
+        return hipSuccess;
+    }
+    // call with mu held, after the plan's launches on s
+    hipError_t after(hipStream_t s)
+    {
+        last = s;
+        used = true;
+        return hipEventRecord(done, s);
+    }
+};
+
 struct gfft_plan_s {
     gfft_desc d{};
     int prec = 0;   // arithmetic: 0 FP32, 1 FP64
@@ -427,6 +460,7 @@ struct gfft_plan_s {
     bool out_has_gaps = false;
     std::mutex host_mu;
     void *hin = nullptr, *hout = nullptr;
+    StreamOrder order;  // used when the plan has a workspace
 };
 
 namespace gfft {
@@ -1029,8 +1063,16 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
 int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
         hipEvent_t *events = nullptr, const float *scale = nullptr)
 {
-    return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events, scale,
-                        p->scale_div);
+    if (!p->scratch)  // no shared workspace: calls on any streams are independent
+        return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events, scale,
+                            p->scale_div);
+    std::lock_guard<std::mutex> lk(p->order.mu);
+    HIP_TRY(p->order.before(s));
+    int st = run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events, scale,
+                          p->scale_div);
+    if (st) return st;
+    HIP_TRY(p->order.after(s));
+    return GFFT_OK;
 }
 
 int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, const void *in, void *out,
@@ -1158,6 +1200,7 @@ struct gfft_batch_plan_s {
     void *g_out = nullptr;
     hipStream_t cap_stream = nullptr;
     std::map<int64_t, gfft_plan> per_job;  // one single-transform plan per length (reference loop)
+    StreamOrder order;  // scratch, counter and the captured graph are shared by all calls
 };
 
 namespace gfft {
@@ -1817,7 +1860,14 @@ int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode
     if (int st = on_plan_device(plan->device)) return st;
     const hipStream_t s = (hipStream_t)stream;
     std::lock_guard<std::mutex> lk(plan->mu);
-    if (mode == GFFT_BATCH_EAGER) return enqueue_batch(plan, in, out, s);
+    if (mode == GFFT_BATCH_EAGER) {
+        std::lock_guard<std::mutex> lo(plan->order.mu);
+        HIP_TRY(plan->order.before(s));
+        int st = enqueue_batch(plan, in, out, s);
+        if (st) return st;
+        HIP_TRY(plan->order.after(s));
+        return GFFT_OK;
+    }
     if (mode == GFFT_BATCH_PER_JOB) {
         // the plain path: one single-transform plan launch per job
         for (int64_t j = 0; j < plan->count; j++) {
@@ -1839,8 +1889,11 @@ int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode
         return GFFT_OK;
     }
     if (mode != GFFT_BATCH_GRAPH) return fail(GFFT_ERR_INVALID_ARG, "unknown batch mode %d", mode);
+    std::lock_guard<std::mutex> lo(plan->order.mu);
     if (!plan->exec || plan->g_in != in || plan->g_out != out) {
         if (plan->exec) {
+            // the previous graph may still run on some stream: let it finish
+            if (plan->order.used) HIP_TRY(hipEventSynchronize(plan->order.done));
             (void)hipGraphExecDestroy(plan->exec);
             (void)hipGraphDestroy(plan->graph);
             plan->exec = nullptr;
@@ -1861,7 +1914,9 @@ int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode
         plan->g_in = in;
         plan->g_out = out;
     }
+    HIP_TRY(plan->order.before(s));
     HIP_TRY(hipGraphLaunch(plan->exec, s));
+    HIP_TRY(plan->order.after(s));
     return GFFT_OK;
 }
 
@@ -1885,6 +1940,7 @@ struct gfft_filter_plan_s {
     void *tw = nullptr;        // W_n^k, k < n/2
     const void *pass_tw = nullptr;  // shared device table (not owned)
     void *partial = nullptr;   // batch x (n/2 + 1) complex
+    StreamOrder order;         // the partial buffer is shared by backward calls
     ~gfft_filter_plan_s()
     {
         if (tw) (void)hipFree(tw);
@@ -1975,9 +2031,13 @@ int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, co
     a.batch = plan->batch;
     a.bins = plan->n / 2 + 1;
     a.vec = ((((uintptr_t)x) | ((uintptr_t)gy) | ((uintptr_t)gx)) & 15) == 0;
-    HIP_TRY(launch_filter(plan->prec, 1, a, (hipStream_t)stream));
+    const hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(plan->order.mu);
+    HIP_TRY(plan->order.before(s));
+    HIP_TRY(launch_filter(plan->prec, 1, a, s));
     a.y = gH;
-    HIP_TRY(launch_filter(plan->prec, 2, a, (hipStream_t)stream));
+    HIP_TRY(launch_filter(plan->prec, 2, a, s));
+    HIP_TRY(plan->order.after(s));
     return GFFT_OK;
 }
 

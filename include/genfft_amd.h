@@ -128,8 +128,10 @@ GFFT_API int gfft_plan_destroy(gfft_plan plan);
  * kind except GFFT_COLS / GFFT_C2C_2D / GFFT_R2C_2D / GFFT_C2R_2D with different
  * pitches (C2R_2D also needs out != in: its output is real).
  * direction: GFFT_FORWARD or GFFT_INVERSE (R2C / R2C_2D / DIT are forward
- * only, C2R inverse only).  Calls that share a plan may not overlap in time
- * (they share the plan's workspace). */
+ * only, C2R inverse only).  Calls may share a plan from any threads and
+ * streams, as with genFFT's shared plans (FFTLevel.h:76-92): a call on a
+ * stream other than the plan's previous one first waits (hipStreamWaitEvent)
+ * for the plan's previous work, because they share its workspace. */
 GFFT_API int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void *stream);
 
 /* gfft_execute for 16-bit storage plans with a per-transform scale:

@@ -158,3 +158,59 @@ def test_linearity_full_size(gpu):
     for r in (0, 77, 127):
         ref = O.fft_ld(x[r].cpu().numpy().astype(np.complex128))
         assert O.rel_l2(fx[r].cpu().numpy(), ref) <= O.tol_c64(n)
+
+
+@pytest.mark.parametrize("kind", ["four_step", "c2r"])
+def test_shared_plan_across_streams_and_threads(gpu, kind):
+    """One plan with a workspace driven from several streams (and host
+    threads) at once gives the same bits as running it serially: calls on a
+    new stream wait for the plan's previous work (genFFT's plans are reentrant
+    across threads for distinct buffers, FFTLevel.h:76-92)."""
+    import threading
+
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    g = torch.Generator(device=gpu).manual_seed(4)
+    if kind == "four_step":
+        n, b = 1 << 20, 8
+        p = G.Plan(C.GFFT_C2C, n, batch=b)
+        xs = [torch.randn(b, n, dtype=torch.complex64, device=gpu, generator=g) for _ in range(4)]
+        ys = [torch.empty_like(x) for x in xs]
+        inv = False
+    else:
+        n, b = 1 << 18, 8
+        p = G.Plan(C.GFFT_C2R, n, batch=b)
+        xs = [G.rfft(torch.randn(b, n, device=gpu, generator=g)) for _ in range(4)]
+        ys = [torch.empty(b, n, device=gpu) for _ in range(4)]
+        inv = True
+    assert p.workspace_bytes() > 0
+    ref = []
+    for x in xs:
+        y = torch.empty_like(ys[0])
+        p.execute(x, y, inverse=inv)
+        torch.cuda.synchronize()
+        ref.append(y.clone())
+    streams = [torch.cuda.Stream(device=gpu) for _ in range(4)]
+    for rep in range(3):
+        for x, y, s in zip(xs, ys, streams):     # back-to-back on four streams, no host sync
+            p.execute(x, y, inverse=inv, stream=s)
+        torch.cuda.synchronize()
+        for y, r in zip(ys, ref):
+            assert torch.equal(y, r)
+        for y in ys:
+            y.zero_()
+        torch.cuda.synchronize()
+
+    def worker(i):
+        for _ in range(5):
+            p.execute(xs[i], ys[i], inverse=inv, stream=streams[i])
+        streams[i].synchronize()
+
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    torch.cuda.synchronize()
+    for y, r in zip(ys, ref):
+        assert torch.equal(y, r)
