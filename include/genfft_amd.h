@@ -200,7 +200,9 @@ GFFT_API int gfft_batch_plan_create(gfft_batch_plan *plan, int precision, int64_
                                     const int64_t *in_offsets, const int64_t *out_offsets,
                                     const int32_t *directions);
 /* Device buffers; asynchronous on `stream`.  The graph is re-captured when in
- * or out change.  Calls on one plan may not overlap in time. */
+ * or out change (after the previous graph has finished).  Calls on different
+ * streams are ordered like gfft_execute's: they share the plan's scratch and
+ * work-queue counter. */
 GFFT_API int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode, void *stream);
 GFFT_API int gfft_batch_plan_describe(gfft_batch_plan plan, char *buf, size_t len);
 GFFT_API int gfft_batch_plan_destroy(gfft_batch_plan plan);
