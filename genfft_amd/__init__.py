@@ -99,6 +99,16 @@ class Plan:
                                                   C.GFFT_SCALE_AUTO if auto else C.GFFT_SCALE_GIVEN,
                                                   s.cuda_stream))
 
+    def execute_scrambled(self, inp: torch.Tensor, out: torch.Tensor, inverse: bool = False, stream=None) -> None:
+        """The plan on input whose elements (C2C) or rows (COLS) are in
+        bit-reversed order (gfft_execute_scrambled); inp may be out."""
+        dev = _check_gpu(inp, out)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        with torch.cuda.device(dev):
+            C.check(self._lib.gfft_execute_scrambled(self._h, inp.data_ptr(), out.data_ptr(),
+                                                     C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
+                                                     s.cuda_stream))
+
     def execute_ptr(self, in_ptr: int, out_ptr: int, inverse: bool = False, stream_handle: int = 0) -> None:
         C.check(self._lib.gfft_execute(self._h, in_ptr, out_ptr,
                                        C.GFFT_INVERSE if inverse else C.GFFT_FORWARD, stream_handle))
@@ -256,6 +266,13 @@ class FFT:
         b = _batch_of(inp, self.n)
         _plan(C.GFFT_C2C, self.n, self._prec, dev, batch=b).execute(inp, out, inverse=inv)
 
+    def transform_no_scramble(self, inout: torch.Tensor, inv: bool = False) -> None:
+        """fft.h:69-73: in place, input already in bit-reversed order."""
+        dev = _check_gpu(inout)
+        _need_contig(inout)
+        b = _batch_of(inout, self.n)
+        _plan(C.GFFT_C2C, self.n, self._prec, dev, batch=b).execute_scrambled(inout, inout, inverse=inv)
+
     def forward(self, out, inp):
         self.transform(out, inp, False)
 
@@ -346,6 +363,12 @@ class FFTVert:
         dev = _check_gpu(out, inp)
         _plan(C.GFFT_COLS, self.n, self._prec, dev, n2=int(cols), in_stride=int(in_stride),
               out_stride=int(out_stride)).execute(inp, out, inverse=inv)
+
+    def transform_no_scramble(self, data, stride, cols, inv: bool = False) -> None:
+        """fft.h:132-136: in place, rows already in bit-reversed order."""
+        dev = _check_gpu(data)
+        _plan(C.GFFT_COLS, self.n, self._prec, dev, n2=int(cols), in_stride=int(stride),
+              out_stride=int(stride)).execute_scrambled(data, data, inverse=inv)
 
 
 class FFT2D:

@@ -47,6 +47,7 @@ EXPORTS = (
     "gfft_execute",
     "gfft_execute_host",
     "gfft_execute_scaled",
+    "gfft_execute_scrambled",
     "gfft_plan_workspace",
     "gfft_plan_describe",
     "gfft_plan_explain",
@@ -121,6 +122,8 @@ def lib() -> ctypes.CDLL:
         L.gfft_plan_destroy.argtypes = [vp]
         L.gfft_execute_scaled.argtypes = [vp, vp, vp, i32, vp, i32, vp]
         L.gfft_execute_scaled.restype = i32
+        L.gfft_execute_scrambled.argtypes = [vp, vp, vp, i32, vp]
+        L.gfft_execute_scrambled.restype = i32
         L.gfft_plan_destroy.restype = i32
         L.gfft_execute.argtypes = [vp, vp, vp, i32, vp]
         L.gfft_execute.restype = i32

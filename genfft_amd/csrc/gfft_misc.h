@@ -75,6 +75,17 @@ struct AmaxArgs {
 
 hipError_t launch_amax_scale(AmaxArgs a, hipStream_t s);
 
+// Bit-reversal gather of rows: for each transform b and row r < n,
+//   out[b*dist + rev(r)*stride + c] = in[b*dist + r*stride + c],  c < width
+// (a 1D transform is rows of one element).  n is a power of two.
+struct BitrevArgs {
+    const void *in;
+    void *out;
+    int64_t n, width, stride, dist, batch;
+    int32_t lg, pad;
+};
+hipError_t launch_bitrev(int prec, const BitrevArgs &a, hipStream_t s);
+
 hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s);
 hipError_t launch_pack(int prec, const UnpackArgs &a, hipStream_t s);
 hipError_t launch_separate(int prec, const SeparateArgs &a, hipStream_t s);

@@ -10,6 +10,7 @@
 //   hfill       RealFFT2D's Hermitian column fill (include/genFFT/FFTReal.h:90-103).
 //   copy        the n = 1 transform (identity on each element).
 //   amax        per-transform power-of-two scale of 16-bit storage input.
+//   bitrev      bit-reversal gather of rows (transform_no_scramble input).
 //
 // All of them are grid-stride loops over (pair, batch) with 64-bit indices
 // and read each input element once.
@@ -271,6 +272,22 @@ __global__ void __launch_bounds__(256) amax_finish_kernel(AmaxArgs a)
         reinterpret_cast<float *>(a.bits)[b] = scale_from_max(__uint_as_float(a.bits[b]), a.gain, a.limit);
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) bitrev_kernel(BitrevArgs a)
+{
+    using C = cx<T>;
+    const C *in = reinterpret_cast<const C *>(a.in);
+    C *out = reinterpret_cast<C *>(a.out);
+    const int64_t per = a.n * a.width, total = per * a.batch;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = g / per, rem = g - b * per;
+        const int64_t r = rem / a.width, c = rem - r * a.width;
+        const int64_t rr = a.lg ? (int64_t)(__brev((uint32_t)r) >> (32 - a.lg)) : 0;
+        out[b * a.dist + rr * a.stride + c] = in[b * a.dist + r * a.stride + c];
+    }
+}
+
 unsigned grid_for(int64_t work)
 {
     int64_t blocks = (work + 255) / 256;
@@ -300,6 +317,14 @@ hipError_t launch_amax_scale(AmaxArgs a, hipStream_t s)
     if (f16) hipLaunchKernelGGL(amax_kernel<st_f16>, dim3((unsigned)(a.batch * a.chunks)), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(amax_kernel<st_bf16>, dim3((unsigned)(a.batch * a.chunks)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(amax_finish_kernel, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bitrev(int prec, const BitrevArgs &a, hipStream_t s)
+{
+    const unsigned g = grid_for(a.n * a.width * a.batch);
+    if (prec) hipLaunchKernelGGL(bitrev_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(bitrev_kernel<float>, dim3(g), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -461,6 +461,7 @@ struct gfft_plan_s {
     std::mutex host_mu;
     void *hin = nullptr, *hout = nullptr;
     StreamOrder order;  // used when the plan has a workspace
+    void *perm = nullptr;  // natural-order copy of a scrambled input (gfft_execute_scrambled)
 };
 
 namespace gfft {
@@ -1677,6 +1678,7 @@ int gfft_plan_destroy(gfft_plan plan)
     if (plan->scratch) (void)hipFree(plan->scratch);
     if (plan->hin) (void)hipFree(plan->hin);
     if (plan->hout) (void)hipFree(plan->hout);
+    if (plan->perm) (void)hipFree(plan->perm);
     delete plan;
     return GFFT_OK;
 }
@@ -1723,6 +1725,47 @@ int gfft_execute_scaled(gfft_plan plan, const void *in, void *out, int direction
         HIP_TRY(launch_amax_scale(a, s));
     }
     return run(plan, in, out, direction, s, nullptr, scales);
+}
+
+int gfft_execute_scrambled(gfft_plan plan, const void *in, void *out, int direction, void *stream)
+{
+    if (!plan) return fail(GFFT_ERR_INVALID_ARG, "null plan");
+    if (!in || !out) return fail(GFFT_ERR_INVALID_ARG, "null buffer");
+    if (direction != GFFT_FORWARD && direction != GFFT_INVERSE)
+        return fail(GFFT_ERR_INVALID_ARG, "direction must be GFFT_FORWARD or GFFT_INVERSE");
+    const gfft_desc &d = plan->d;
+    if ((d.kind != GFFT_C2C && d.kind != GFFT_COLS) || plan->store)
+        return fail(GFFT_ERR_UNSUPPORTED, "scrambled input needs a complex64 / complex128 C2C or COLS plan");
+    if (int st = on_plan_device(plan->device)) return st;
+    BitrevArgs a{};
+    a.n = d.n;
+    a.lg = ilog2(d.n);
+    a.batch = d.batch;
+    if (d.kind == GFFT_C2C) {
+        a.width = 1;
+        a.stride = 1;
+        a.dist = dflt(d.in_dist, d.n);
+    } else {
+        a.width = d.n2;
+        a.stride = dflt(d.in_stride, d.n2);
+        a.dist = dflt(d.in_dist, d.n * a.stride);
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(plan->order.mu);
+    if (!plan->perm) {
+        hipError_t e = hipMalloc(&plan->perm, plan->in_bytes);
+        if (e != hipSuccess) return fail(GFFT_ERR_ALLOC, "scramble buffer of %zu bytes: %s", plan->in_bytes,
+                                         hipGetErrorString(e));
+    }
+    HIP_TRY(plan->order.before(s));
+    a.in = in;
+    a.out = plan->perm;
+    HIP_TRY(launch_bitrev(plan->prec, a, s));
+    int st = run_launches(plan->launches, plan->prec, plan->xcd_remap, plan->perm, out, plan->scratch, direction, s,
+                          nullptr, nullptr, plan->scale_div);
+    if (st) return st;
+    HIP_TRY(plan->order.after(s));
+    return GFFT_OK;
 }
 
 int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)

@@ -149,6 +149,15 @@ GFFT_API int gfft_execute(gfft_plan plan, const void *in, void *out, int directi
 GFFT_API int gfft_execute_scaled(gfft_plan plan, const void *in, void *out, int direction, float *scales,
                                  int scale_mode, void *stream);
 
+/* gfft_execute on input in bit-reversed order: element j of each transform
+ * (C2C) or row j of each matrix (COLS) holds what the natural order puts at
+ * bit-reverse(j).  Replaces FFT<T>::transform_no_scramble
+ * (include/genFFT/fft.h:69-73) and FFTVert<T>::transform_no_scramble
+ * (fft.h:132-136) on device data: the rows are gathered into natural order in
+ * a plan-owned buffer (allocated on first use), then the plan runs; in == out
+ * is allowed.  complex64 / complex128 C2C and COLS plans. */
+GFFT_API int gfft_execute_scrambled(gfft_plan plan, const void *in, void *out, int direction, void *stream);
+
 /* Same as gfft_execute for host buffers: stages through plan-owned device
  * memory and returns after the result is back in `out` (synchronous). */
 GFFT_API int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction);

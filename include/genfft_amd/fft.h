@@ -167,12 +167,15 @@ struct FFT {
     }
 
     /// fft.h:69-73: in-place transform of data already in bit-reversed order
-    /// (host memory only: device-resident data is never bit-reversed here).
+    /// (device data: gathered into natural order on the device first).
     template <bool inv>
     void transform_no_scramble(std::complex<T> *inout)
     {
-        if (gfft_is_device_pointer(inout))
-            throw error(GFFT_ERR_UNSUPPORTED, "genfft_amd: transform_no_scramble takes host memory");
+        if (gfft_is_device_pointer(inout)) {
+            detail::check(gfft_execute_scrambled(plan->h, inout, inout, inv ? GFFT_INVERSE : GFFT_FORWARD,
+                                                 detail::stream_ref()));
+            return;
+        }
         std::vector<std::complex<T>> nat(n);
         for (int i = 0; i < n; i++) nat[detail::bitrev(i, n)] = inout[i];
         detail::check(gfft_execute_host(plan->h, nat.data(), inout, inv ? GFFT_INVERSE : GFFT_FORWARD));
@@ -226,12 +229,16 @@ struct FFTVert {
         transform<inv>(out, cols, in, cols, cols);
     }
 
-    /// fft.h:132-136: rows already in bit-reversed order (host memory only).
+    /// fft.h:132-136: rows already in bit-reversed order (host or device data).
     template <bool inv>
     void transform_no_scramble(std::complex<T> *data, stride_t stride, index_t cols)
     {
-        if (gfft_is_device_pointer(data))
-            throw error(GFFT_ERR_UNSUPPORTED, "genfft_amd: transform_no_scramble takes host memory");
+        if (gfft_is_device_pointer(data)) {
+            auto p = detail::get_plan(GFFT_COLS, detail::precision<T>(), n, cols, 1, stride, stride, 0, 0, 0);
+            detail::check(gfft_execute_scrambled(p->h, data, data, inv ? GFFT_INVERSE : GFFT_FORWARD,
+                                                 detail::stream_ref()));
+            return;
+        }
         std::vector<std::complex<T>> nat((size_t)n * stride);
         for (int r = 0; r < n; r++)
             std::memcpy(&nat[(size_t)detail::bitrev(r, n) * stride], data + (size_t)r * stride,

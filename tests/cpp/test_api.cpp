@@ -29,6 +29,13 @@
 extern "C" void orc_fft_ld(const double *in, double *out, int64_t n, int64_t batch, int inv);
 
 static int g_fail = 0, g_pass = 0;
+
+static int genfft_amd_bitrev(int i, int n)
+{
+    int r = 0;
+    for (int m = n >> 1; m; m >>= 1, i >>= 1) r = (r << 1) | (i & 1);
+    return r;
+}
 #define EXPECT(cond, ...)                                               \
     do {                                                                \
         if (!(cond)) {                                                  \
@@ -306,6 +313,27 @@ void TestDevicePointers(int n, int batch)
     g_pass++;
 }
 
+// FFT<T>::transform_no_scramble (fft.h:69-73) on device memory: the input is
+// stored in bit-reversed order, the output comes back in natural order.
+template <typename T>
+void TestNoScrambleDevice(int n)
+{
+    std::vector<std::complex<T>> in(n), scr(n), out(n);
+    DummyData(in, false);
+    for (int i = 0; i < n; i++) scr[genfft_amd_bitrev(i, n)] = in[i];
+    void *d = nullptr;
+    const size_t bytes = (size_t)n * sizeof(std::complex<T>);
+    EXPECT(hipMalloc(&d, bytes) == hipSuccess, "hipMalloc");
+    EXPECT(hipMemcpy(d, scr.data(), bytes, hipMemcpyHostToDevice) == hipSuccess, "H2D");
+    genfft::FFT<T> fft(n);
+    fft.template transform_no_scramble<false>((std::complex<T> *)d);
+    EXPECT(hipMemcpy(out.data(), d, bytes, hipMemcpyDeviceToHost) == hipSuccess, "D2H");
+    (void)hipFree(d);
+    auto ref = oracle(in.data(), n, false);
+    EXPECT(rel_l2(out.data(), ref.data(), n) <= RelTol<T>(n), "transform_no_scramble device n=%d", n);
+    g_pass++;
+}
+
 void TestErrors()
 {
     bool threw = false;
@@ -358,6 +386,10 @@ int main(int argc, char **argv)
     }
     TestDevicePointers<float>(1 << 20, 4);
     TestDevicePointers<double>(4096, 16);
+    for (int n : {1, 2, 16, 1024, 1 << 18}) {
+        TestNoScrambleDevice<float>(n);
+        TestNoScrambleDevice<double>(n);
+    }
     TestErrors();
     std::printf("genfft C++ API tests: %d passed, %d failed\n", g_pass, g_fail);
     return g_fail ? 1 : 0;

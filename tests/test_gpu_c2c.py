@@ -214,3 +214,44 @@ def test_shared_plan_across_streams_and_threads(gpu, kind):
     torch.cuda.synchronize()
     for y, r in zip(ys, ref):
         assert torch.equal(y, r)
+
+
+def _bitrev_perm(n):
+    lg = n.bit_length() - 1
+    return np.array([int(format(i, f"0{lg}b")[::-1], 2) if lg else 0 for i in range(n)])
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [1, 2, 8, 1024, 1 << 14, 1 << 20])
+def test_transform_no_scramble_on_device(gpu, prec, n):
+    """FFT<T>::transform_no_scramble (fft.h:69-73) on device data: input in
+    bit-reversed order, in place, natural-order output -- the same bits as the
+    natural-order transform of the unscrambled data."""
+    import genfft_amd as G
+    cdt = torch.complex64 if prec == "f32" else torch.complex128
+    b = 3
+    g = torch.Generator(device=gpu).manual_seed(n)
+    x = torch.randn(b, n, dtype=cdt, device=gpu, generator=g)
+    perm = torch.from_numpy(_bitrev_perm(n)).to(gpu)
+    scr = torch.empty_like(x)
+    scr[:, perm] = x
+    for inv in (False, True):
+        want = G.fft(x, inverse=inv)
+        got = scr.clone()
+        G.FFT(n, cdt).transform_no_scramble(got, inv)
+        assert torch.equal(got, want)
+
+
+@pytest.mark.parametrize("n,cols,stride", [(8, 7, 9), (1024, 33, 40), (4096, 16, 16)])
+def test_vert_no_scramble_on_device(gpu, n, cols, stride):
+    """FFTVert<T>::transform_no_scramble (fft.h:132-136): rows in bit-reversed order."""
+    import genfft_amd as G
+    g = torch.Generator(device=gpu).manual_seed(cols)
+    x = torch.randn(n, stride, dtype=torch.complex64, device=gpu, generator=g)
+    want = torch.zeros_like(x)
+    G.FFTVert(n, torch.complex64).transform(want, stride, x, stride, cols)
+    perm = torch.from_numpy(_bitrev_perm(n)).to(gpu)
+    scr = x.clone()
+    scr[perm] = x
+    G.FFTVert(n, torch.complex64).transform_no_scramble(scr, stride, cols)
+    assert torch.equal(scr[:, :cols], want[:, :cols])
