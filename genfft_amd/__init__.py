@@ -27,6 +27,7 @@ import torch
 
 from . import _capi as C
 from ._capi import GfftError
+from ._launch import call_on as _call_on, device_index as _dev_idx, stream_handle as _stream_handle
 
 __all__ = [
     "Plan", "FFT", "FFTVert", "DIT", "FFT2D", "RealFFT", "RealFFT2D", "UnevenBatch",
@@ -78,11 +79,8 @@ class Plan:
 
     def execute(self, inp: torch.Tensor, out: torch.Tensor, inverse: bool = False, stream=None) -> None:
         dev = _check_gpu(inp, out)
-        s = stream if stream is not None else torch.cuda.current_stream(dev)
-        with torch.cuda.device(dev):
-            C.check(self._lib.gfft_execute(self._h, inp.data_ptr(), out.data_ptr(),
-                                           C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
-                                           s.cuda_stream))
+        _call_on(_dev_idx(dev), stream, self._lib.gfft_execute, self._h, inp.data_ptr(), out.data_ptr(),
+                 C.GFFT_INVERSE if inverse else C.GFFT_FORWARD)
 
     def execute_scaled(self, inp: torch.Tensor, out: torch.Tensor, scales: torch.Tensor, auto: bool,
                        inverse: bool = False, stream=None) -> None:
@@ -91,23 +89,16 @@ class Plan:
         dev = _check_gpu(inp, out, scales)
         if scales.dtype != torch.float32 or not scales.is_contiguous() or scales.numel() < self.desc.batch:
             raise ValueError("scales must be a contiguous float32 tensor of batch elements")
-        s = stream if stream is not None else torch.cuda.current_stream(dev)
-        with torch.cuda.device(dev):
-            C.check(self._lib.gfft_execute_scaled(self._h, inp.data_ptr(), out.data_ptr(),
-                                                  C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
-                                                  scales.data_ptr(),
-                                                  C.GFFT_SCALE_AUTO if auto else C.GFFT_SCALE_GIVEN,
-                                                  s.cuda_stream))
+        _call_on(_dev_idx(dev), stream, self._lib.gfft_execute_scaled, self._h, inp.data_ptr(), out.data_ptr(),
+                 C.GFFT_INVERSE if inverse else C.GFFT_FORWARD, scales.data_ptr(),
+                 C.GFFT_SCALE_AUTO if auto else C.GFFT_SCALE_GIVEN)
 
     def execute_scrambled(self, inp: torch.Tensor, out: torch.Tensor, inverse: bool = False, stream=None) -> None:
         """The plan on input whose elements (C2C) or rows (COLS) are in
         bit-reversed order (gfft_execute_scrambled); inp may be out."""
         dev = _check_gpu(inp, out)
-        s = stream if stream is not None else torch.cuda.current_stream(dev)
-        with torch.cuda.device(dev):
-            C.check(self._lib.gfft_execute_scrambled(self._h, inp.data_ptr(), out.data_ptr(),
-                                                     C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
-                                                     s.cuda_stream))
+        _call_on(_dev_idx(dev), stream, self._lib.gfft_execute_scrambled, self._h, inp.data_ptr(), out.data_ptr(),
+                 C.GFFT_INVERSE if inverse else C.GFFT_FORWARD)
 
     def execute_ptr(self, in_ptr: int, out_ptr: int, inverse: bool = False, stream_handle: int = 0) -> None:
         C.check(self._lib.gfft_execute(self._h, in_ptr, out_ptr,
@@ -117,13 +108,13 @@ class Plan:
         """Runs once, returning the per-launch kernel durations in ms (HIP events
         recorded on the execution stream around each launch)."""
         dev = _check_gpu(inp, out)
-        s = stream if stream is not None else torch.cuda.current_stream(dev)
         ms = (ctypes.c_float * 4096)()
         cnt = ctypes.c_int()
         with torch.cuda.device(dev):
             C.check(self._lib.gfft_plan_profile(self._h, inp.data_ptr(), out.data_ptr(),
                                                 C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
-                                                s.cuda_stream, ms, len(ms), ctypes.byref(cnt)))
+                                                _stream_handle(_dev_idx(dev), stream), ms, len(ms),
+                                                ctypes.byref(cnt)))
         return [ms[i] for i in range(min(cnt.value, len(ms)))]
 
     def workspace_bytes(self) -> int:
@@ -177,10 +168,8 @@ class UnevenBatch:
 
     def execute(self, inp: torch.Tensor, out: torch.Tensor, mode: str = "graph", stream=None) -> None:
         dev = _check_gpu(inp, out)
-        s = stream if stream is not None else torch.cuda.current_stream(dev)
-        with torch.cuda.device(dev):
-            C.check(self._lib.gfft_batch_execute(self._h, inp.data_ptr(), out.data_ptr(), self.MODES[mode],
-                                                 s.cuda_stream))
+        _call_on(_dev_idx(dev), stream, self._lib.gfft_batch_execute, self._h, inp.data_ptr(), out.data_ptr(),
+                 self.MODES[mode])
 
     def describe(self) -> str:
         buf = ctypes.create_string_buffer(1 << 20)
@@ -291,11 +280,8 @@ class FFT:
         dev = _check_gpu(out, in1, in2)
         _need_contig(out, in1, in2)
         b = _batch_of(in1, self.n)
-        lib = C.lib()
-        s = torch.cuda.current_stream(dev).cuda_stream
-        with torch.cuda.device(dev):
-            C.check(lib.gfft_interleave_real(self._prec, out.data_ptr(), in1.data_ptr(), in2.data_ptr(),
-                                             self.n, b, s))
+        _call_on(_dev_idx(dev), None, C.lib().gfft_interleave_real, self._prec, out.data_ptr(), in1.data_ptr(),
+                 in2.data_ptr(), self.n, b)
         _plan(C.GFFT_C2C, self.n, self._prec, dev, batch=b).execute(out, out)
 
 
@@ -425,10 +411,8 @@ def separate_2x_real_FFT(out1: torch.Tensor, out2: torch.Tensor, inp: torch.Tens
     """FFTReal.h:36-66 on device data (batched over rows of n)."""
     dev = _check_gpu(out1, out2, inp)
     b = _batch_of(inp, n)
-    s = torch.cuda.current_stream(dev).cuda_stream
-    with torch.cuda.device(dev):
-        C.check(C.lib().gfft_separate_2x_real(_prec(inp.dtype), out1.data_ptr(), out2.data_ptr(),
-                                              inp.data_ptr(), n, b, s))
+    _call_on(_dev_idx(dev), None, C.lib().gfft_separate_2x_real, _prec(inp.dtype), out1.data_ptr(),
+             out2.data_ptr(), inp.data_ptr(), n, b)
 
 
 # ---------------------------------------------------------------------------

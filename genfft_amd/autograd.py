@@ -22,6 +22,7 @@ import threading
 import torch
 
 from . import _capi as C
+from ._launch import call_on
 
 _plans: dict = {}
 _lock = threading.Lock()
@@ -31,34 +32,47 @@ class _FilterPlan:
     def __init__(self, prec: int, n: int, batch: int, device: torch.device):
         self._lib = C.lib()
         self._h = ctypes.c_void_p()
-        with torch.cuda.device(device):
+        self.index = device.index if device.index is not None else torch.cuda.current_device()
+        with torch.cuda.device(self.index):
             C.check(self._lib.gfft_filter_plan_create(ctypes.byref(self._h), prec, n, batch))
+        self.handle = self._h.value
+        self.forward = self._lib.gfft_filter_forward
+        self.backward = self._lib.gfft_filter_backward
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self._lib.gfft_filter_plan_destroy(self._h)
 
 
-def _plan(x: torch.Tensor) -> _FilterPlan:
-    prec = {torch.float32: C.GFFT_F32, torch.float64: C.GFFT_F64}[x.dtype]
-    n = x.shape[-1]
-    batch = x.numel() // n
-    key = (prec, n, batch, x.device)
-    with _lock:
-        p = _plans.get(key)
-        if p is None:
-            p = _plans[key] = _FilterPlan(prec, n, batch, x.device)
-        return p
+_COMPLEX_OF = {torch.float32: torch.complex64, torch.float64: torch.complex128}
+_PREC_OF = {torch.float32: C.GFFT_F32, torch.float64: C.GFFT_F64}
 
 
 def _check(x: torch.Tensor, H: torch.Tensor):
     if not (x.is_cuda and H.is_cuda):
         raise ValueError("spectral_filter runs on GPU tensors only")
-    want = {torch.float32: torch.complex64, torch.float64: torch.complex128}.get(x.dtype)
+    want = _COMPLEX_OF.get(x.dtype)
     if want is None or H.dtype != want:
         raise TypeError("x must be float32/float64 and H the matching complex dtype")
     if H.shape[-1] != x.shape[-1] // 2 + 1 or H.dim() != 1:
         raise ValueError("H must hold n/2 + 1 bins")
+    if H.device != x.device:
+        raise ValueError("x and H must be on the same device")
+
+
+def _plan(x: torch.Tensor, H: torch.Tensor) -> _FilterPlan:
+    """Validated plan for this (dtype, shape, device); arguments are checked once per key."""
+    key = (x.dtype, x.shape, x.device, H.dtype, H.shape, H.device)
+    p = _plans.get(key)
+    if p is not None:
+        return p
+    _check(x, H)
+    n = x.shape[-1]
+    with _lock:
+        p = _plans.get(key)
+        if p is None:
+            p = _plans[key] = _FilterPlan(_PREC_OF[x.dtype], n, x.numel() // n, x.device)
+        return p
 
 
 class SpectralFilter(torch.autograd.Function):
@@ -66,28 +80,24 @@ class SpectralFilter(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, x: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
-        _check(x, H)
+        p = _plan(x, H)
         x = x.contiguous()
         H = H.contiguous()
         y = torch.empty_like(x)
-        p = _plan(x)
-        s = torch.cuda.current_stream(x.device).cuda_stream
-        with torch.cuda.device(x.device):
-            C.check(p._lib.gfft_filter_forward(p._h, x.data_ptr(), H.data_ptr(), y.data_ptr(), 0, s))
+        call_on(p.index, None, p.forward, p.handle, x.data_ptr(), H.data_ptr(), y.data_ptr(), 0)
         ctx.save_for_backward(x, H)
+        ctx.plan = p
         return y
 
     @staticmethod
     def backward(ctx, gy: torch.Tensor):
         x, H = ctx.saved_tensors
+        p = ctx.plan
         gy = gy.contiguous()
         gx = torch.empty_like(x)
         gH = torch.empty_like(H)
-        p = _plan(x)
-        s = torch.cuda.current_stream(x.device).cuda_stream
-        with torch.cuda.device(x.device):
-            C.check(p._lib.gfft_filter_backward(p._h, x.data_ptr(), H.data_ptr(), gy.data_ptr(), gx.data_ptr(),
-                                                gH.data_ptr(), s))
+        call_on(p.index, None, p.backward, p.handle, x.data_ptr(), H.data_ptr(), gy.data_ptr(), gx.data_ptr(),
+                gH.data_ptr())
         return gx, gH
 
 

@@ -176,6 +176,20 @@ def test_filter_plan_argument_checks():
         assert lib.gfft_filter_plan_create(ctypes.byref(h), C.GFFT_F32, 1024, 4) == C.GFFT_ERR_NO_DEVICE
 
 
+def test_spectral_filter_rejects_host_tensors():
+    """The autograd wrapper validates before it plans, and never caches a failed key."""
+    import torch
+
+    import genfft_amd as G
+    from genfft_amd import autograd as A
+    x, H = torch.zeros(4, 64), torch.zeros(33, dtype=torch.complex64)
+    for fn in (G.spectral_filter, G.spectral_filter_plain):
+        for _ in range(2):
+            with pytest.raises(ValueError, match="GPU tensors only"):
+                fn(x, H)
+    assert not any(k[2].type == "cpu" for k in A._plans)
+
+
 @pytest.mark.parametrize("prec", [C.GFFT_F16, C.GFFT_BF16])
 def test_16bit_storage_plans_in_bounds(prec):
     """FP16 / BF16 storage: C2C 2..2^23, 2D (incl. 8192^2) and the COLS table

@@ -191,6 +191,24 @@ def test_filter_in_place_and_batch_shapes(gpu):
     assert O.rel_l2(y, ref) <= 2 * O.tol_c64(512)
 
 
+def test_filter_strided_inputs_and_gradients(gpu):
+    """Non-contiguous x, H and upstream gradient give the contiguous results."""
+    import genfft_amd as G
+    g = torch.Generator(device=gpu).manual_seed(4)
+    xt = torch.randn(512, 8, device=gpu, generator=g)
+    Hs = torch.randn(2 * 257, dtype=torch.complex64, device=gpu, generator=g)[::2]
+    gyt = torch.randn(512, 8, device=gpu, generator=g)
+    outs = []
+    for x, H, gy in ((xt.t(), Hs, gyt.t()), (xt.t().contiguous(), Hs.contiguous(), gyt.t().contiguous())):
+        x = x.detach().requires_grad_(True)
+        H = H.detach().requires_grad_(True)
+        y = G.spectral_filter(x, H)
+        y.backward(gy)
+        outs.append((y.detach(), x.grad, H.grad))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
 def test_filter_rejects_bad_arguments(gpu):
     import genfft_amd as G
     from genfft_amd import GfftError
