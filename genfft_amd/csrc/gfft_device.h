@@ -755,6 +755,21 @@ __global__ void __launch_bounds__(NT) tile_fft_kernel_h(TileArgs a)
 // and the four-step twiddle bases of tile t are issued before the prefetch
 // of tile t+1 for the same reason (vector-memory counters drain in order).
 // ---------------------------------------------------------------------------
+// Thread index of a persistent loop, rebuilt on every iteration from the
+// wave's index (uniform, kept in an SGPR) and a lane id that v_mbcnt
+// computes afresh.  The volatile asm cannot be hoisted, so no VGPR stays
+// live across the loop to hold the index and nothing derived from it is
+// kept either: with tid itself live, the column kernel of the 2^20
+// four-step spilled it, and the scratch reload at the loop head waited for
+// every outstanding store and prefetch (vmcnt counts all of them, in order)
+// before the tile's four-step twiddle loads were even issued.
+__device__ __forceinline__ int fresh_tid(int wave)
+{
+    int lane;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+    return (wave << 6) + lane;
+}
+
 template <typename T, int L, int M, int NT, class G, int INP>
 __device__ __forceinline__ bool fetch_tile(cx<T> (&reg)[M * L / NT], const TileArgs &a, int o2, int o, int s0,
                                            int nvalid, int tid)
@@ -851,12 +866,12 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
         decode(tile, o2, o, s0, nvalid);
         vec = fetch_tile<T, L, M, NT, G, INP>(reg, a, o2, o, s0, nvalid, tid);
     }
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     while (tile < a.n_tiles) {
-        // Re-materialise the thread index every iteration: without this the
-        // compiler hoists every tid-derived LDS offset of every pass out of
-        // the loop and keeps them live, which costs ~100 VGPRs.
-        int t = tid;
-        asm volatile("" : "+v"(t));
+        // The thread index is rebuilt every iteration: otherwise the compiler
+        // hoists every tid-derived LDS offset of every pass out of the loop
+        // and keeps them live, which costs ~100 VGPRs.
+        const int t = fresh_tid(wave);
         // four-step twiddle bases of this tile (s-fast mapping: s = t % M):
         // issued first, so their latency hides behind the wait for the
         // prefetched tile data that put_tile needs anyway
@@ -1011,10 +1026,10 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
         decode(tile, o2, o, s0, nvalid);
         vec = fetch_tile_h<S, L, M, NT, G, INP>(reg, a, o2, o, s0, nvalid, tid);
     }
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     while (tile < a.n_tiles) {
-        // thread index re-materialised per iteration (see tile_fft_pipe_kernel)
-        int t = tid;
-        asm volatile("" : "+v"(t));
+        // thread index rebuilt per iteration (see tile_fft_pipe_kernel)
+        const int t = fresh_tid(wave);
         cx<double> hj{}, lj{}, hd{}, ld{}, hr{}, lr{};
         const bool tw = LAYOUT == LAY_ELEM && NT % M == 0 && a.tw_mode != TW_NONE;
         if (tw) {

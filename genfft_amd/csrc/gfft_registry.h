@@ -26,8 +26,11 @@ struct KernelEntry {
     int store;       // 0: complex T in memory; 1: FP16 pairs; 2: BF16 pairs (FP32 arithmetic)
 };
 
+// Shape of a tile entry, without its kernel: each make_* below takes the
+// address of exactly the kernel it registers, so no translation unit
+// instantiates kernels of another table.
 template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
-constexpr KernelEntry make_entry()
+constexpr KernelEntry entry_shape()
 {
     static_assert(((RS * ...) == L), "radices must multiply to L");
     static_assert((M * L) % NT == 0, "tile must split evenly over threads");
@@ -48,6 +51,14 @@ constexpr KernelEntry make_entry()
     e.lds_bytes = (int)(G::ELEMS * sizeof(cx<T>));
     e.persistent = 0;
     e.tune_only = 0;
+    e.fn = nullptr;
+    return e;
+}
+
+template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
+constexpr KernelEntry make_entry()
+{
+    KernelEntry e = entry_shape<T, L, M, NT, LAY, INP, OUTP, RS...>();
     e.fn = &tile_fft_kernel<T, L, M, NT, LAY, INP, OUTP, RS...>;
     return e;
 }
@@ -63,14 +74,21 @@ constexpr KernelEntry make_variant()
 // Persistent, software-pipelined variant (same tile math, LDS-resident pass
 // twiddles, next tile prefetched into registers).
 template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
-constexpr KernelEntry make_pipe_entry()
+constexpr KernelEntry pipe_shape()
 {
-    KernelEntry e = make_entry<T, L, M, NT, LAY, INP, OUTP, RS...>();
+    KernelEntry e = entry_shape<T, L, M, NT, LAY, INP, OUTP, RS...>();
     using G = TileGeom<L, M, LAY, INP, OUTP>;
     constexpr int twn = pass_tw_count<1, RS...>();
     static_assert((G::ELEMS + (twn > 0 ? twn : 1)) * sizeof(cx<T>) <= 160 * 1024, "tile + twiddles exceed LDS");
     e.lds_bytes = (int)((G::ELEMS + (twn > 0 ? twn : 1)) * sizeof(cx<T>));
     e.persistent = 1;
+    return e;
+}
+
+template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
+constexpr KernelEntry make_pipe_entry()
+{
+    KernelEntry e = pipe_shape<T, L, M, NT, LAY, INP, OUTP, RS...>();
     e.fn = &tile_fft_pipe_kernel<T, L, M, NT, LAY, INP, OUTP, RS...>;
     return e;
 }
@@ -79,7 +97,7 @@ constexpr KernelEntry make_pipe_entry()
 template <class S, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
 constexpr KernelEntry make_entry_h()
 {
-    KernelEntry e = make_entry<float, L, M, NT, LAY, INP, OUTP, RS...>();
+    KernelEntry e = entry_shape<float, L, M, NT, LAY, INP, OUTP, RS...>();
     e.lds_bytes += M * (int)sizeof(float);
     e.store = std::is_same<S, st_f16>::value ? 1 : 2;
     e.fn = &tile_fft_kernel_h<S, L, M, NT, LAY, INP, OUTP, RS...>;
@@ -90,7 +108,7 @@ constexpr KernelEntry make_entry_h()
 template <class S, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
 constexpr KernelEntry make_pipe_entry_h()
 {
-    KernelEntry e = make_pipe_entry<float, L, M, NT, LAY, INP, OUTP, RS...>();
+    KernelEntry e = pipe_shape<float, L, M, NT, LAY, INP, OUTP, RS...>();
     e.lds_bytes += M * (int)sizeof(float);
     e.store = std::is_same<S, st_f16>::value ? 1 : 2;
     e.fn = &tile_fft_pipe_kernel_h<S, L, M, NT, LAY, INP, OUTP, RS...>;
