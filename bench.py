@@ -87,7 +87,7 @@ def parse():
 def tile_launches(text):
     out = []
     for ln in text.splitlines():
-        if not ln.startswith(("tile", "unpack", "pack", "hfill", "copy")):
+        if not ln.startswith(("tile", "r2c_rows_unpack", "unpack", "pack", "hfill", "copy")):
             continue
         f = dict(kv.split("=", 1) for kv in ln.split()[1:] if "=" in kv)
         f["kind"] = ln.split()[0]
@@ -390,6 +390,10 @@ def main():
             key = f"tile L={l['L']} M={l['M']} {'row' if l['inp'] == '0' else 'col'}->" \
                   f"{'row' if l['outp'] == '0' else 'col'}"
             elems = int(l["n_inner"]) * int(l["tiles"]) // int(l["tiles_inner"]) * int(l["L"])
+        elif l["kind"] == "r2c_rows_unpack":
+            # reads the n/2-point column-pass output, writes n/2 + 1 bins per transform
+            key = f"r2c rows+unpack R={l['R']} M={l['M']}"
+            elems = int(l["tiles"]) // (int(l["n1"]) // int(l["M"])) * int(l["n1"]) * int(l["R"])
         else:
             key, elems = l["kind"], 0
         g = groups.setdefault(key, {"launches": 0, "ms": 0.0, "bytes": 0.0})

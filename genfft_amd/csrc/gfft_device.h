@@ -437,25 +437,26 @@ __device__ __forceinline__ void load_tile(cx<T> *lds, const TileArgs &a, int o2,
             return;
         }
     }
-#pragma unroll 4
-    for (int it = 0; it < (TOT + NT - 1) / NT; it++) {
+    // Element by element (complex128, odd strides, partial tiles): every
+    // load is issued into registers before the first LDS write.  Written
+    // interleaved, the compiler orders each load after the previous LDS
+    // write (the input pointer is generic) and the loads run one at a time.
+    constexpr int IT = (TOT + NT - 1) / NT;
+    cx<T> r[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
         const int e = tid + it * NT;
-        if (TOT % NT == 0 || e < TOT) {
-            int s, n;
-            if constexpr (INP == PAT_ROW) {
-                s = e / L;
-                n = e % L;
-            } else {
-                n = e / M;
-                s = e % M;
-            }
-            if (s < nvalid) {
-                const cx<T> *p = INP == PAT_ROW ? in + (int64_t)s * a.in_inner + n
-                                                : in + (int64_t)n * a.in_elem + s;
-                cx<T> x = *p;
-                lds[G::off(s, n)] = sw ? swap_ri(x) : x;
-            }
-        }
+        const int s = INP == PAT_ROW ? e / L : e % M;
+        const int n = INP == PAT_ROW ? e % L : e / M;
+        if ((TOT % NT == 0 || e < TOT) && s < nvalid)
+            r[it] = INP == PAT_ROW ? in[(int64_t)s * a.in_inner + n] : in[(int64_t)n * a.in_elem + s];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int e = tid + it * NT;
+        const int s = INP == PAT_ROW ? e / L : e % M;
+        const int n = INP == PAT_ROW ? e % L : e / M;
+        if ((TOT % NT == 0 || e < TOT) && s < nvalid) lds[G::off(s, n)] = sw ? swap_ri(r[it]) : r[it];
     }
 }
 

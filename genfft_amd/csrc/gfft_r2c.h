@@ -1,0 +1,42 @@
+// gfft_r2c.h -- the fused last pass of a four-step real-input FFT.
+//
+// A real signal of n = 2L samples is transformed as the L-point complex FFT
+// Z of its (even, odd) pairs, then unpacked into the n/2 + 1 bins F
+// (FFTReal.h:204-213 with adjust_DIT_impl, generic/fft_dit_impl_generic.inl:27-61):
+//   F_k = E_k - i W_n^k O_k,  F_{L-k} = conj(E_k + i W_n^k O_k),
+//   E_k = (Z_k + conj Z_{L-k}) / 2,  O_k = (Z_k - conj Z_{L-k}) / 2.
+// For L = n1 * R the four-step column pass leaves row k_c of Z's second
+// pass in scratch; bin k = k_c + n1 * k_r comes out of row k_c, element k_r,
+// and its partner L - k out of row n1 - k_c, element R - 1 - k_r.  One
+// workgroup takes M/2 rows and their M/2 mirror rows, so the unpack pairs
+// meet in LDS and the bins are written straight to the output: the
+// separate unpack pass over HBM disappears.
+#pragma once
+
+#include "gfft_misc.h"
+#include "gfft_registry.h"
+
+namespace gfft {
+
+struct R2cArgs {
+    const void *in;    // row r of transform b at in + b * in_dist + r * R (complex elements)
+    void *out;         // bins 0 .. L of transform b at out + b * out_dist
+    int64_t in_dist, out_dist;
+    int32_t n1;        // rows per transform (the column pass length)
+    int32_t tiles_per; // workgroups per transform: n1 / M
+    int32_t n_tiles;   // tiles_per * transforms of this launch
+    int32_t vec_in;    // 16-byte loads allowed (complex64, alignment proven on host)
+    int32_t xcd_remap;
+    int32_t pad;
+    const void *pass_tw;  // Stockham pass twiddles of the R-point row FFT
+    TwTable tw;           // W_n, n = 2L the real length
+};
+
+// Row length R, tile rows M (half of them mirrors), threads and radices of
+// the fused kernel for a precision (0 FP32, 1 FP64); the entry has no
+// tile-kernel function, only the shape (for the pass twiddle table).
+const KernelEntry *r2c_fused_entry(int prec);
+
+hipError_t launch_r2c_fused(int prec, const R2cArgs &a, hipStream_t s);  // one workgroup per tile
+
+}  // namespace gfft

@@ -41,6 +41,7 @@
 #include "gfft_batch.h"
 #include "gfft_filter.h"
 #include "gfft_misc.h"
+#include "gfft_r2c.h"
 #include "gfft_registry.h"
 
 namespace gfft {
@@ -273,6 +274,7 @@ struct PlanOpts {
     int row_m = 0;         // preferred transposing-row tile width (0: widest)
     int pipe_mask = 3;     // pipelined kernels allowed: bit 0 column tiles, bit 1 transposing rows
     int row_variant = -1;  // registry index of the single-pass row kernel (-1: default)
+    int r2c_fuse = 1;      // half-spectrum real FFTs: fused row-and-unpack pass where the shape allows
 };
 
 // Picks the tile kernel for (precision, L, patterns): the registry entry
@@ -377,6 +379,7 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
     if (getenv("GFFT_COL_M")) o.col_m = (int)env_int("GFFT_COL_M", 0);
     if (getenv("GFFT_ROW_M")) o.row_m = (int)env_int("GFFT_ROW_M", 0);
     if (getenv("GFFT_PIPE")) o.pipe_mask = (int)env_int("GFFT_PIPE", 3);
+    if (getenv("GFFT_R2C_FUSE")) o.r2c_fuse = (int)env_int("GFFT_R2C_FUSE", 1);
     return o;
 }
 
@@ -384,7 +387,7 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
 // Plan representation
 // ---------------------------------------------------------------------------
 enum BufId { B_IN = 0, B_OUT = 1, B_SCR = 2 };
-enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET };
+enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET, LT_R2C };
 
 // bytes of one complex element in memory
 size_t csize_of(int prec, int store) { return store ? 4 : (prec ? 16 : 8); }
@@ -397,6 +400,7 @@ struct Launch {
     HFillArgs ha{};
     CopyArgs ca{};
     MixArgs mix{};
+    R2cArgs ra{};
     int in_buf = B_IN, out_buf = B_OUT;
     int64_t in_off = 0, out_off = 0;  // bytes
     bool vin_ok = false, vout_ok = false;
@@ -677,6 +681,56 @@ struct Builder {
         p->launches.push_back(l);
         return GFFT_OK;
     }
+
+    // Whether the half-spectrum real FFT of length n runs as a column pass
+    // plus the fused row-and-unpack pass: n/2 = n1 * R with the n1-point
+    // column pass in the table and whole workgroups of row pairs.
+    bool r2c_fusable(int64_t n) const
+    {
+        const KernelEntry *fe = r2c_fused_entry(prec);
+        const int64_t L = n / 2, R = fe->L;
+        if (store || !is_pow2(n) || L <= max_len(prec, PAT_ROW, PAT_ROW) || L % R) return false;
+        const int64_t n1 = L / R;
+        return n1 >= fe->M && n1 % fe->M == 0 && n1 <= max_len(prec, PAT_COL, PAT_COL);
+    }
+
+    // Half-spectrum real FFT of length n (n/2 = n1 * R): the n1-point
+    // column pass of the n/2-point four-step into scratch, then one launch
+    // that runs the R-point row FFTs and writes bins 0 .. n/2 of the output.
+    int r2c_fused(int64_t n, int64_t batch, int64_t in_dist, int64_t out_dist)
+    {
+        const KernelEntry *fe = r2c_fused_entry(prec);
+        const int64_t L = n / 2, R = fe->L, n1 = L / R;
+        const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
+        const int64_t chunk = std::min(batch, std::max<int64_t>(1, chunk_bytes / (L * (int64_t)csize)));
+        need_scratch(chunk * L);
+        for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
+            const int64_t cb = std::min(chunk, batch - b0);
+            int st = tile((int)n1, PAT_COL, PAT_COL, 0, B_IN, b0 * (in_dist / 2), B_SCR, 0, R, cb, 1, 0, in_dist / 2, 1,
+                          R, 0, L, 1, R, TW_INNER, L, true, false);
+            if (st) return st;
+            Launch l;
+            l.type = LT_R2C;
+            l.k = fe;
+            R2cArgs &r = l.ra;
+            r.in_dist = L;
+            r.out_dist = out_dist;
+            r.n1 = (int32_t)n1;
+            r.tiles_per = (int32_t)(n1 / fe->M);
+            if ((int64_t)r.tiles_per * cb > INT32_MAX) return fail(GFFT_ERR_UNSUPPORTED, "too many tiles");
+            r.n_tiles = (int32_t)(r.tiles_per * cb);
+            if ((st = get_pass_twiddles(p->device, *fe, &r.pass_tw, dry))) return st;
+            if ((st = get_two_level(p->device, n, &r.tw, dry))) return st;
+            l.in_buf = B_SCR;
+            l.out_buf = B_OUT;
+            l.out_off = b0 * out_dist * (int64_t)csize;
+            l.vin_ok = prec == 0;  // rows start at even element offsets of scratch
+            l.last = true;
+            l.grid = r.n_tiles;
+            p->launches.push_back(l);
+        }
+        return GFFT_OK;
+    }
 };
 
 int64_t dflt(int64_t v, int64_t d) { return v ? v : d; }
@@ -717,6 +771,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             break;
         }
         if (id % 2) return fail(GFFT_ERR_UNSUPPORTED, "R2C input distance must be even");
+        if (half && b.r2c_fusable(n) && b.opts.r2c_fuse) {
+            st = b.r2c_fused(n, batch, id, od);
+            break;
+        }
         if (od % 2 == 0) {
             st = b.c2c_1d(n / 2, batch, B_IN, 0, id / 2, B_OUT, 0, od, true, true);
             if (!st) st = b.unpack(n, batch, B_OUT, od, B_OUT, od, half, false);
@@ -970,6 +1028,16 @@ int validate(const gfft_plan_s *p)
                 return st;
             break;
         }
+        case LT_R2C: {
+            const R2cArgs &a = l.ra;
+            const int64_t nb = a.n_tiles / a.tiles_per, L = (int64_t)a.n1 * l.k->L;
+            if ((int64_t)a.tiles_per * nb != a.n_tiles || (int64_t)a.tiles_per * l.k->M != a.n1)
+                return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d r2c tiles", idx);
+            if ((st = inside(l.in_buf, l.in_off, ((nb - 1) * a.in_dist + L) * cs, "r2c row load"))) return st;
+            if ((st = inside(l.out_buf, l.out_off, ((nb - 1) * a.out_dist + L + 1) * cs, "r2c bin store"))) return st;
+            if (l.k->lds_bytes > 160 * 1024) return fail(GFFT_ERR_UNSUPPORTED, "internal: LDS");
+            break;
+        }
         case LT_HFILL: {
             const HFillArgs &a = l.ha;
             if ((st = inside(l.out_buf, l.out_off,
@@ -1029,6 +1097,13 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
                      "half=%d\n", l.type == LT_PACK ? "pack" : "unpack", (long long)l.ua.n, (long long)l.ua.batch,
                      bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off,
                      (long long)l.ua.in_dist, (long long)l.ua.out_dist, l.ua.half);
+            break;
+        case LT_R2C:
+            snprintf(line, sizeof(line), "r2c_rows_unpack R=%d M=%d NT=%d n1=%d tiles=%d grid=%lld in=%s+%lld out=%s+%lld "
+                     "in_dist=%lld out_dist=%lld\n", l.k->L, l.k->M, l.k->NT, l.ra.n1, l.ra.n_tiles, (long long)l.grid,
+                     bn[l.in_buf],
+                     (long long)l.in_off, bn[l.out_buf], (long long)l.out_off, (long long)l.ra.in_dist,
+                     (long long)l.ra.out_dist);
             break;
         case LT_HFILL:
             snprintf(line, sizeof(line), "hfill M=%lld N=%lld vcols=%lld stride=%lld dist=%lld batch=%lld\n",
@@ -1122,6 +1197,15 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             HFillArgs a = l.ha;
             a.out = op;
             HIP_TRY(launch_hfill(p->prec, a, s));
+            break;
+        }
+        case LT_R2C: {
+            R2cArgs a = l.ra;
+            a.in = ip;
+            a.out = op;
+            a.vec_in = l.vin_ok && (((uintptr_t)ip & 15) == 0);
+            a.xcd_remap = p->xcd_remap;
+            HIP_TRY(launch_r2c_fused(p->prec, a, s));
             break;
         }
         case LT_COPY: {

@@ -242,6 +242,33 @@ def test_c2r_2d_rejections():
 
 
 @pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
+def test_r2c_half_fuses_the_unpack_into_the_row_pass(prec, monkeypatch):
+    """Half-spectrum R2C of a four-step length n/2 = n1 * 512 runs two launches
+    (column pass, fused row-and-unpack pass) per chunk; full spectra, single
+    pass lengths and n1 beyond the column table keep the separate unpack.  The
+    bounds proof holds for dense (odd), padded and chunked outputs."""
+    maxrow = 16384 if prec == C.GFFT_F32 else 8192
+    for k in range(1, 24):
+        n = 1 << k
+        L = n // 2
+        fused = L > maxrow and L // 512 <= 2048
+        for od in (0, n // 2 + 2, n // 2 + 7):
+            for b in (1, 4):
+                st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=n, batch=b, outd=od, flags=C.GFFT_FLAG_HALF))
+                assert st == C.GFFT_OK, (n, od, b, C.lib().gfft_last_error())
+                assert ("r2c_rows_unpack" in text) == fused, (n, text)
+                assert ("unpack n=" in text) == (not fused and n > 1), (n, text)
+        st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=n, batch=2))
+        assert st == C.GFFT_OK and "r2c_rows_unpack" not in text
+    monkeypatch.setenv("GFFT_CHUNK_BYTES", str(2 * (1 << 19) * (8 if prec == C.GFFT_F32 else 16)))
+    st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=1 << 20, batch=5, flags=C.GFFT_FLAG_HALF))
+    assert st == C.GFFT_OK and text.count("r2c_rows_unpack") == 3 and text.count("tile ") == 3
+    monkeypatch.setenv("GFFT_R2C_FUSE", "0")
+    st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=1 << 20, batch=5, flags=C.GFFT_FLAG_HALF))
+    assert st == C.GFFT_OK and "r2c_rows_unpack" not in text and "unpack n=" in text
+
+
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
 def test_dit_plans_for_multiples_of_four(prec):
     """DIT<T>(n) takes any n divisible by 4, as genFFT's dynamic FFTDIT<-1>
     (generic/fft_dit_impl_generic.inl:117-153); other kinds stay powers of two."""

@@ -164,6 +164,65 @@ def test_real_full_size_properties(gpu):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("lg", range(15, 22))
+@pytest.mark.parametrize("pad", [0, 3])
+def test_real_half_fused_unpack_vs_oracle(gpu, prec, lg, pad):
+    """Half spectra of four-step lengths, where the unpack runs inside the
+    last pass (row k_c and its mirror n1 - k_c in one workgroup): every bin
+    of every transform against the oracle, dense (odd distance n/2 + 1) and
+    padded output; the pad between spectra stays untouched."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    n, b = 1 << lg, 3
+    nout = n // 2 + 1
+    od = nout + pad
+    rng = np.random.default_rng(lg * 10 + pad)
+    x = rng.standard_normal((b, n)).astype(rdt)
+    out = torch.full((b * od,), 43 + 21j, dtype=tcdt, device=gpu)
+    p = G.Plan(C.GFFT_R2C, n, precision=0 if prec == "f32" else 1, batch=b, half=True, out_dist=od)
+    fused = n // 2 > (16384 if prec == "f32" else 8192)
+    assert ("r2c_rows_unpack" in p.describe()) == fused
+    p.execute(torch.from_numpy(x).to(gpu), out)
+    y = out.cpu().numpy().reshape(b, od)
+    ref = O.fft_ld(x.astype(np.complex128))[:, :nout]
+    assert O.rel_l2_rows(y[:, :nout], ref).max() <= _tol(prec, n)
+    assert np.all(y[:, nout:] == 43 + 21j), "write between spectra"
+    assert np.all(y[:, 0].imag == 0) and np.all(y[:, nout - 1].imag == 0)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_real_half_fused_matches_unfused_and_chunks(gpu, prec, monkeypatch):
+    """N = 2^20: the fused plan, a plan split into three chunks through a
+    small scratch, and the separate-unpack plan agree to rounding; spot rows
+    against the oracle."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    n, b = 1 << 20, 5
+    g = torch.Generator(device=gpu).manual_seed(7)
+    x = torch.randn(b, n, device=gpu, generator=g, dtype=trdt)
+    outs = {}
+    for name, env in (("fused", {}), ("chunked", {"GFFT_CHUNK_BYTES": str(2 * (n // 2) * (8 if prec == "f32" else 16))}),
+                      ("unfused", {"GFFT_R2C_FUSE": "0"})):
+        with monkeypatch.context() as m:
+            for k, v in env.items():
+                m.setenv(k, v)
+            p = G.Plan(C.GFFT_R2C, n, precision=0 if prec == "f32" else 1, batch=b, half=True)
+        text = p.describe()
+        assert ("r2c_rows_unpack" in text) == (name != "unfused")
+        assert text.count("r2c_rows_unpack") == {"fused": 1, "chunked": 3, "unfused": 0}[name]
+        y = torch.empty(b, n // 2 + 1, dtype=tcdt, device=gpu)
+        p.execute(x, y)
+        outs[name] = y.cpu().numpy()
+    tol = _tol(prec, n)
+    for name in ("chunked", "unfused"):
+        assert O.rel_l2_rows(outs[name], outs["fused"]).max() <= tol
+    ref = O.fft_ld(x[[0, b - 1]].cpu().numpy().astype(np.complex128))[:, : n // 2 + 1]
+    assert O.rel_l2_rows(outs["fused"][[0, b - 1]], ref).max() <= tol
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("n", [4, 64, 4096])
 def test_c2r_ignores_imag_of_dc_and_nyquist(gpu, prec, n):
     """A c2r reads bins 0 and n/2 as real (the Hermitian convention): random
