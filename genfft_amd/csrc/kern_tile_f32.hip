@@ -46,6 +46,7 @@ const KernelEntry kTable[] = {
     make_entry<T, 1024, 16, 1024, E_, C_, C_, 16, 16, 4>(),
     make_entry<T, 1024, 8, 512, E_, C_, C_, 16, 16, 4>(),
     make_entry<T, 2048, 8, 1024, E_, C_, C_, 16, 16, 8>(),
+    make_entry<T, 4096, 4, 1024, E_, C_, C_, 16, 16, 16>(),  // 32-byte rows: column transforms only, not four-step
     // --- transposing rows ---
     make_entry<T, 64, 64, 256, S_, R_, C_, 8, 8>(),
     make_entry<T, 128, 32, 256, S_, R_, C_, 16, 8>(),

@@ -319,6 +319,22 @@ int max_len(int prec, int inp, int outp, int store = 0)
     return best;
 }
 
+// Longest column tile whose rows are at least 64 bytes, the limit of the
+// four-step column pass.  The 4096-point complex64 column tile (4 columns,
+// 32-byte rows) serves column transforms of 4096 rows (the 2D 4096 x 4096
+// column pass: 0.082 ms against 0.093 ms for two 64-point sub-passes), but
+// a 4096 x 1024 four-step of 2^22 ran 1.165 ms against 1.115 ms for
+// 2048 x 2048 on MI355X.
+int max_fourstep_col(int prec, int store = 0)
+{
+    int cnt = 0, best = 0;
+    const KernelEntry *tab = tile_table(prec, store, &cnt);
+    const int cs = store ? 4 : (prec ? 16 : 8);
+    for (int i = 0; i < cnt; i++)
+        if (tab[i].inp == PAT_COL && tab[i].outp == PAT_COL && tab[i].M * cs >= 64) best = std::max(best, tab[i].L);
+    return best;
+}
+
 int64_t env_int(const char *name, int64_t dflt)
 {
     const char *v = getenv(name);
@@ -337,7 +353,7 @@ void four_step_split(int prec, int lg, int *l1, int *l2, int n1_override = 0)
     // 2 GiB complex64 batches: 2^16 0.79 ms, 2^18 0.81 ms, 2^20 0.92 ms; for
     // 1 GiB complex128 batches: 2^14 1.04 -> 0.81 ms, 2^18 1.16 -> 0.86 ms
     // against the even split.
-    if (lg - 10 >= 1 && ((int64_t)1 << (lg - 10)) <= max_len(prec, PAT_COL, PAT_COL)) {
+    if (lg - 10 >= 1 && ((int64_t)1 << (lg - 10)) <= max_fourstep_col(prec)) {
         b = 10;
         a = lg - 10;
     }
@@ -691,7 +707,7 @@ struct Builder {
         const int64_t L = n / 2, R = fe->L;
         if (store || !is_pow2(n) || L <= max_len(prec, PAT_ROW, PAT_ROW) || L % R) return false;
         const int64_t n1 = L / R;
-        return n1 >= fe->M && n1 % fe->M == 0 && n1 <= max_len(prec, PAT_COL, PAT_COL);
+        return n1 >= fe->M && n1 % fe->M == 0 && n1 <= max_fourstep_col(prec);
     }
 
     // Half-spectrum real FFT of length n (n/2 = n1 * R): the n1-point

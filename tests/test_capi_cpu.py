@@ -268,6 +268,24 @@ def test_r2c_half_fuses_the_unpack_into_the_row_pass(prec, monkeypatch):
     assert st == C.GFFT_OK and "r2c_rows_unpack" not in text and "unpack n=" in text
 
 
+def test_4096_column_tile_serves_columns_not_four_step():
+    """complex64: 4096-row column transforms (2D 4096^2, FFTVert) run one
+    4-column tile pass; the four-step of 2^22 keeps 2048 x 2048 and the fused
+    real FFT of 2^22 keeps the separate unpack (n1 = 4096 would need it)."""
+    st, text = explain(desc(kind=C.GFFT_C2C_2D, n=4096, n2=4096))
+    t = tiles(text)
+    assert st == C.GFFT_OK and len(t) == 2
+    assert field(t[1], "L") == "4096" and field(t[1], "M") == "4" and field(t[1], "inp") == "1"
+    st, text = explain(desc(kind=C.GFFT_COLS, n=4096, n2=999, ins=1001, outs=1000))
+    t = tiles(text)
+    assert st == C.GFFT_OK and len(t) == 1 and field(t[0], "L") == "4096"
+    st, text = explain(desc(n=1 << 22, batch=32))
+    t = tiles(text)
+    assert st == C.GFFT_OK and [field(x, "L") for x in t] == ["2048", "2048"]
+    st, text = explain(desc(kind=C.GFFT_R2C, n=1 << 22, batch=2, flags=C.GFFT_FLAG_HALF))
+    assert st == C.GFFT_OK and "r2c_rows_unpack" not in text
+
+
 @pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
 def test_dit_plans_for_multiples_of_four(prec):
     """DIT<T>(n) takes any n divisible by 4, as genFFT's dynamic FFTDIT<-1>
