@@ -756,6 +756,16 @@ __global__ void __launch_bounds__(NT) tile_fft_kernel_h(TileArgs a)
 // and the four-step twiddle bases of tile t are issued before the prefetch
 // of tile t+1 for the same reason (vector-memory counters drain in order).
 // ---------------------------------------------------------------------------
+// Pass twiddles of a pipelined tile are copied to LDS when the tile and the
+// table fit a CU's 160 KB together; otherwise (a single 16384-point
+// sequence: 136 KB of tile, 128 KB of table) the passes read them from
+// global memory, where the table stays in L2.
+template <typename T, int ELEMS, int TWN>
+constexpr bool pass_tw_in_lds()
+{
+    return (ELEMS + (TWN > 0 ? TWN : 1)) * sizeof(cx<T>) <= 160 * 1024;
+}
+
 // Thread index of a persistent loop, rebuilt on every iteration from the
 // wave's index (uniform, kept in an SGPR) and a lane id that v_mbcnt
 // computes afresh.  The volatile asm cannot be hoisted, so no VGPR stays
@@ -843,12 +853,14 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
     constexpr int TWN = pass_tw_count<1, RS...>();
     constexpr int E = M * L / NT;
     constexpr int LAST_R = last_radix<RS...>();
+    constexpr bool TWL = pass_tw_in_lds<T, G::ELEMS, TWN>();
     __shared__ cx<T> lds[G::ELEMS];
-    __shared__ cx<T> twl[TWN > 0 ? TWN : 1];
+    __shared__ cx<T> twl[TWL && TWN > 0 ? TWN : 1];
     const int tid = threadIdx.x;
-    {
-        const cx<T> *gtw = reinterpret_cast<const cx<T> *>(a.pass_tw);
-        for (int i = tid; i < TWN; i += NT) twl[i] = gtw[i];
+    const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
+    if constexpr (TWL) {
+        for (int i = tid; i < TWN; i += NT) twl[i] = ptw[i];
+        ptw = twl;
     }
     const int grid = gridDim.x;
     const int slot = a.xcd_remap ? xcd_tile(blockIdx.x, grid) : (int)blockIdx.x;
@@ -909,7 +921,7 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
             vec = fetch_tile<T, L, M, NT, G, INP>(reg, a, no2, no, ns0, nnv, t);
         }
         stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, (LAYOUT == LAY_ELEM && NT % M == 0), 1, RS...>(
-            lds, twl, t, a, s0, o, pre);
+            lds, ptw, t, a, s0, o, pre);
         store_tile<T, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, t);
         __syncthreads();
         tile = next;
@@ -1002,13 +1014,15 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
     constexpr int TWN = pass_tw_count<1, RS...>();
     constexpr int E = M * L / NT;
     constexpr int LAST_R = last_radix<RS...>();
+    constexpr bool TWL = pass_tw_in_lds<T, G::ELEMS + M / 2, TWN>();
     __shared__ cx<T> lds[G::ELEMS];
-    __shared__ cx<T> twl[TWN > 0 ? TWN : 1];
+    __shared__ cx<T> twl[TWL && TWN > 0 ? TWN : 1];
     __shared__ float scl[M];
     const int tid = threadIdx.x;
-    {
-        const cx<T> *gtw = reinterpret_cast<const cx<T> *>(a.pass_tw);
-        for (int i = tid; i < TWN; i += NT) twl[i] = gtw[i];
+    const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
+    if constexpr (TWL) {
+        for (int i = tid; i < TWN; i += NT) twl[i] = ptw[i];
+        ptw = twl;
     }
     const int grid = gridDim.x;
     const int slot = a.xcd_remap ? xcd_tile(blockIdx.x, grid) : (int)blockIdx.x;
@@ -1070,7 +1084,7 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
             vec = fetch_tile_h<S, L, M, NT, G, INP>(reg, a, no2, no, ns0, nnv, t);
         }
         stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, (LAYOUT == LAY_ELEM && NT % M == 0), 1, RS...>(
-            lds, twl, t, a, s0, o, pre);
+            lds, ptw, t, a, s0, o, pre);
         store_tile_h<S, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, t, a.scale ? scl : nullptr);
         __syncthreads();
         tile = next;

@@ -79,8 +79,8 @@ constexpr KernelEntry pipe_shape()
     KernelEntry e = entry_shape<T, L, M, NT, LAY, INP, OUTP, RS...>();
     using G = TileGeom<L, M, LAY, INP, OUTP>;
     constexpr int twn = pass_tw_count<1, RS...>();
-    static_assert((G::ELEMS + (twn > 0 ? twn : 1)) * sizeof(cx<T>) <= 160 * 1024, "tile + twiddles exceed LDS");
-    e.lds_bytes = (int)((G::ELEMS + (twn > 0 ? twn : 1)) * sizeof(cx<T>));
+    constexpr bool twl = pass_tw_in_lds<T, G::ELEMS, twn>();
+    e.lds_bytes = (int)((G::ELEMS + (twl ? (twn > 0 ? twn : 1) : 1)) * sizeof(cx<T>));
     e.persistent = 1;
     return e;
 }

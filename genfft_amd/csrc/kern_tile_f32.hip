@@ -62,6 +62,7 @@ const KernelEntry kTable[] = {
     // one workgroup per CU and lose to the plain kernels ---
     make_pipe_entry<T, 1024, 16, 1024, E_, C_, C_, 8, 8, 16>(),  // (16,16,4): ~2% slower, 12 B/lane spill
     make_pipe_entry<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
+    make_pipe_entry<T, 16384, 1, 1024, S_, R_, R_, 16, 16, 16, 4>(),  // pass twiddles from global (tile: 136 KB of LDS)
     // --- autotuner alternatives: radix order, tile width, threads per tile ---
     make_variant<T, 256, 32, 512, S_, R_, R_, 16, 16>(),
     make_variant<T, 512, 8, 256, S_, R_, R_, 16, 8, 4>(),
