@@ -478,6 +478,11 @@ struct gfft_plan_s {
     int num_cus = 256;
     size_t in_bytes = 0, out_bytes = 0;
     bool out_has_gaps = false;
+    // A chunked plan writes the output of early batch members before it
+    // reads the input of later ones; in_step / out_step are the batch
+    // distances in bytes (1D kinds), see check_alias.
+    bool chunked = false;
+    int64_t in_step = 0, out_step = 0;
     std::mutex host_mu;
     void *hin = nullptr, *hout = nullptr;
     StreamOrder order;  // used when the plan has a workspace
@@ -622,6 +627,7 @@ struct Builder {
         const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
         int64_t chunk = std::max<int64_t>(1, chunk_bytes / (n * (int64_t)csize));
         chunk = std::min(chunk, batch);
+        if (chunk < batch) p->chunked = true;
         need_scratch(chunk * n);
         const int pm1 = opts.col_m, pm2 = opts.row_m;
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
@@ -711,40 +717,38 @@ struct Builder {
     }
 
     // Half-spectrum real FFT of length n (n/2 = n1 * R): the n1-point
-    // column pass of the n/2-point four-step into scratch, then one launch
-    // that runs the R-point row FFTs and writes bins 0 .. n/2 of the output.
+    // column pass of the n/2-point four-step over the whole batch into
+    // scratch, then one launch that runs the R-point row FFTs and writes bins
+    // 0 .. n/2 of the output.  Never chunked: every input element is read
+    // before the first bin is written, so in == out is safe whatever the
+    // distances (with the dense layout a transform's bins overlap the next
+    // transform's input).
     int r2c_fused(int64_t n, int64_t batch, int64_t in_dist, int64_t out_dist)
     {
         const KernelEntry *fe = r2c_fused_entry(prec);
         const int64_t L = n / 2, R = fe->L, n1 = L / R;
-        const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
-        const int64_t chunk = std::min(batch, std::max<int64_t>(1, chunk_bytes / (L * (int64_t)csize)));
-        need_scratch(chunk * L);
-        for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
-            const int64_t cb = std::min(chunk, batch - b0);
-            int st = tile((int)n1, PAT_COL, PAT_COL, 0, B_IN, b0 * (in_dist / 2), B_SCR, 0, R, cb, 1, 0, in_dist / 2, 1,
-                          R, 0, L, 1, R, TW_INNER, L, true, false);
-            if (st) return st;
-            Launch l;
-            l.type = LT_R2C;
-            l.k = fe;
-            R2cArgs &r = l.ra;
-            r.in_dist = L;
-            r.out_dist = out_dist;
-            r.n1 = (int32_t)n1;
-            r.tiles_per = (int32_t)(n1 / fe->M);
-            if ((int64_t)r.tiles_per * cb > INT32_MAX) return fail(GFFT_ERR_UNSUPPORTED, "too many tiles");
-            r.n_tiles = (int32_t)(r.tiles_per * cb);
-            if ((st = get_pass_twiddles(p->device, *fe, &r.pass_tw, dry))) return st;
-            if ((st = get_two_level(p->device, n, &r.tw, dry))) return st;
-            l.in_buf = B_SCR;
-            l.out_buf = B_OUT;
-            l.out_off = b0 * out_dist * (int64_t)csize;
-            l.vin_ok = prec == 0;  // rows start at even element offsets of scratch
-            l.last = true;
-            l.grid = r.n_tiles;
-            p->launches.push_back(l);
-        }
+        need_scratch(batch * L);
+        int st = tile((int)n1, PAT_COL, PAT_COL, 0, B_IN, 0, B_SCR, 0, R, batch, 1, 0, in_dist / 2, 1, R, 0, L, 1, R,
+                      TW_INNER, L, true, false);
+        if (st) return st;
+        Launch l;
+        l.type = LT_R2C;
+        l.k = fe;
+        R2cArgs &r = l.ra;
+        r.in_dist = L;
+        r.out_dist = out_dist;
+        r.n1 = (int32_t)n1;
+        r.tiles_per = (int32_t)(n1 / fe->M);
+        if ((int64_t)r.tiles_per * batch > INT32_MAX) return fail(GFFT_ERR_UNSUPPORTED, "too many tiles");
+        r.n_tiles = (int32_t)(r.tiles_per * batch);
+        if ((st = get_pass_twiddles(p->device, *fe, &r.pass_tw, dry))) return st;
+        if ((st = get_two_level(p->device, n, &r.tw, dry))) return st;
+        l.in_buf = B_SCR;
+        l.out_buf = B_OUT;
+        l.vin_ok = prec == 0;  // rows start at even element offsets of scratch
+        l.last = true;
+        l.grid = r.n_tiles;
+        p->launches.push_back(l);
         return GFFT_OK;
     }
 };
@@ -768,6 +772,8 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         p->in_bytes = ((batch - 1) * id + n) * cs;
         p->out_bytes = ((batch - 1) * od + n) * cs;
         p->out_has_gaps = od > n;
+        p->in_step = id * (int64_t)cs;
+        p->out_step = od * (int64_t)cs;
         st = b.c2c_1d(n, batch, B_IN, 0, id, B_OUT, 0, od, true, true);
         break;
     }
@@ -778,6 +784,8 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         p->in_bytes = ((batch - 1) * id + n) * rs;
         p->out_bytes = ((batch - 1) * od + nout) * cs;
         p->out_has_gaps = od > nout;
+        p->in_step = id * (int64_t)rs;
+        p->out_step = od * (int64_t)cs;
         p->dir_mask = 1;
         if (n == 1) {
             Launch l;
@@ -787,7 +795,12 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             break;
         }
         if (id % 2) return fail(GFFT_ERR_UNSUPPORTED, "R2C input distance must be even");
-        if (half && b.r2c_fusable(n) && b.opts.r2c_fuse) {
+        // The fused path needs scratch for the whole batch.  The separate
+        // unpack with an odd output distance needs the same; with an even
+        // one it chunks the transform through at most GFFT_CHUNK_BYTES, so a
+        // larger batch keeps that path.
+        const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
+        if (half && b.r2c_fusable(n) && b.opts.r2c_fuse && (od % 2 || batch * (n / 2) * (int64_t)cs <= chunk_bytes)) {
             st = b.r2c_fused(n, batch, id, od);
             break;
         }
@@ -1783,6 +1796,20 @@ int gfft_plan_destroy(gfft_plan plan)
     return GFFT_OK;
 }
 
+// A chunked plan (a batch larger than its workspace) reads the input of
+// later batch members after writing the output of earlier ones, so
+// overlapping buffers are only safe when every transform is in place.
+static int check_alias(const gfft_plan_s *p, const void *in, const void *out)
+{
+    if (!p->chunked) return GFFT_OK;
+    const uintptr_t i0 = (uintptr_t)in, o0 = (uintptr_t)out;
+    const bool overlap = i0 < o0 + p->out_bytes && o0 < i0 + p->in_bytes;
+    if (!overlap || (i0 == o0 && p->in_step == p->out_step)) return GFFT_OK;
+    return fail(GFFT_ERR_UNSUPPORTED,
+                "this plan runs its batch in chunks: overlapping in/out buffers need in == out and equal "
+                "input and output distances in bytes (or a batch within GFFT_CHUNK_BYTES)");
+}
+
 int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void *stream)
 {
     if (!plan) return fail(GFFT_ERR_INVALID_ARG, "null plan");
@@ -1793,6 +1820,7 @@ int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void 
         return fail(GFFT_ERR_INVALID_ARG, "this plan kind supports only the %s direction",
                     plan->dir_mask == 1 ? "forward" : "inverse");
     if (int st = on_plan_device(plan->device)) return st;
+    if (int st = check_alias(plan, in, out)) return st;
     return run(plan, in, out, direction, (hipStream_t)stream);
 }
 
@@ -1807,6 +1835,7 @@ int gfft_execute_scaled(gfft_plan plan, const void *in, void *out, int direction
     if (scale_mode != GFFT_SCALE_GIVEN && scale_mode != GFFT_SCALE_AUTO)
         return fail(GFFT_ERR_INVALID_ARG, "scale_mode must be GFFT_SCALE_GIVEN or GFFT_SCALE_AUTO");
     if (int st = on_plan_device(plan->device)) return st;
+    if (int st = check_alias(plan, in, out)) return st;
     hipStream_t s = (hipStream_t)stream;
     if (scale_mode == GFFT_SCALE_AUTO) {
         AmaxArgs a{};
@@ -1893,6 +1922,7 @@ int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int direction, 
 {
     if (!plan || !in || !out || !count) return fail(GFFT_ERR_INVALID_ARG, "null argument");
     if (int st = on_plan_device(plan->device)) return st;
+    if (int st = check_alias(plan, in, out)) return st;
     const int n = (int)plan->launches.size();
     *count = n;
     std::vector<hipEvent_t> ev(2 * n);

@@ -126,7 +126,11 @@ GFFT_API int gfft_plan_destroy(gfft_plan plan);
 /* Executes on device-resident buffers, asynchronously on `stream`
  * (a hipStream_t; NULL = the null stream).  in == out is allowed for every
  * kind except GFFT_COLS / GFFT_C2C_2D / GFFT_R2C_2D / GFFT_C2R_2D with different
- * pitches (C2R_2D also needs out != in: its output is real).
+ * pitches (C2R_2D also needs out != in: its output is real).  A 1D plan whose
+ * batch exceeds its workspace (1 GiB, or GFFT_CHUNK_BYTES) runs the batch in
+ * chunks and then accepts overlapping buffers only as in == out with equal
+ * input and output distances in bytes; other overlaps return
+ * GFFT_ERR_UNSUPPORTED.
  * direction: GFFT_FORWARD or GFFT_INVERSE (R2C / R2C_2D / DIT are forward
  * only, C2R inverse only).  Calls may share a plan from any threads and
  * streams, as with genFFT's shared plans (FFTLevel.h:76-92): a call on a

@@ -260,9 +260,18 @@ def test_r2c_half_fuses_the_unpack_into_the_row_pass(prec, monkeypatch):
                 assert ("unpack n=" in text) == (not fused and n > 1), (n, text)
         st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=n, batch=2))
         assert st == C.GFFT_OK and "r2c_rows_unpack" not in text
+    # the fused plan never chunks (every input is read before a bin is
+    # written); past the chunk size an even output distance keeps the
+    # chunked separate-unpack path, an odd one (full scratch either way) fuses
     monkeypatch.setenv("GFFT_CHUNK_BYTES", str(2 * (1 << 19) * (8 if prec == C.GFFT_F32 else 16)))
     st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=1 << 20, batch=5, flags=C.GFFT_FLAG_HALF))
-    assert st == C.GFFT_OK and text.count("r2c_rows_unpack") == 3 and text.count("tile ") == 3
+    assert st == C.GFFT_OK and text.count("r2c_rows_unpack") == 1 and text.count("tile ") == 1
+    st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=1 << 20, batch=5, outd=(1 << 19) + 2,
+                            flags=C.GFFT_FLAG_HALF))
+    assert st == C.GFFT_OK and "r2c_rows_unpack" not in text and text.count("tile ") == 6
+    st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=1 << 20, batch=2, outd=(1 << 19) + 2,
+                            flags=C.GFFT_FLAG_HALF))
+    assert st == C.GFFT_OK and "r2c_rows_unpack" in text
     monkeypatch.setenv("GFFT_R2C_FUSE", "0")
     st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=1 << 20, batch=5, flags=C.GFFT_FLAG_HALF))
     assert st == C.GFFT_OK and "r2c_rows_unpack" not in text and "unpack n=" in text

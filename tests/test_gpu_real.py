@@ -193,33 +193,80 @@ def test_real_half_fused_unpack_vs_oracle(gpu, prec, lg, pad):
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_real_half_fused_matches_unfused_and_chunks(gpu, prec, monkeypatch):
-    """N = 2^20: the fused plan, a plan split into three chunks through a
-    small scratch, and the separate-unpack plan agree to rounding; spot rows
-    against the oracle."""
+    """N = 2^20: the fused plan and the separate-unpack plan (whole batch, and
+    chunked through a small scratch with an even output distance) agree to
+    rounding; spot rows against the oracle."""
     import genfft_amd as G
     from genfft_amd import _capi as C
     rdt, cdt, trdt, tcdt = _dts(prec)
     n, b = 1 << 20, 5
+    nout = n // 2 + 1
     g = torch.Generator(device=gpu).manual_seed(7)
     x = torch.randn(b, n, device=gpu, generator=g, dtype=trdt)
+    small = str(2 * (n // 2) * (8 if prec == "f32" else 16))
     outs = {}
-    for name, env in (("fused", {}), ("chunked", {"GFFT_CHUNK_BYTES": str(2 * (n // 2) * (8 if prec == "f32" else 16))}),
-                      ("unfused", {"GFFT_R2C_FUSE": "0"})):
+    for name, env, od, want in (("fused", {}, nout, "r2c_rows_unpack"),
+                                ("unfused", {"GFFT_R2C_FUSE": "0"}, nout, "unpack n="),
+                                ("chunked", {"GFFT_CHUNK_BYTES": small}, nout + 1, "unpack n=")):
         with monkeypatch.context() as m:
             for k, v in env.items():
                 m.setenv(k, v)
-            p = G.Plan(C.GFFT_R2C, n, precision=0 if prec == "f32" else 1, batch=b, half=True)
-        text = p.describe()
-        assert ("r2c_rows_unpack" in text) == (name != "unfused")
-        assert text.count("r2c_rows_unpack") == {"fused": 1, "chunked": 3, "unfused": 0}[name]
-        y = torch.empty(b, n // 2 + 1, dtype=tcdt, device=gpu)
+            p = G.Plan(C.GFFT_R2C, n, precision=0 if prec == "f32" else 1, batch=b, half=True, out_dist=od)
+        assert want in p.describe(), name
+        y = torch.empty(b, od, dtype=tcdt, device=gpu)
         p.execute(x, y)
-        outs[name] = y.cpu().numpy()
+        outs[name] = y[:, :nout].cpu().numpy()
+    assert outs["chunked"].shape == outs["fused"].shape
     tol = _tol(prec, n)
     for name in ("chunked", "unfused"):
         assert O.rel_l2_rows(outs[name], outs["fused"]).max() <= tol
-    ref = O.fft_ld(x[[0, b - 1]].cpu().numpy().astype(np.complex128))[:, : n // 2 + 1]
+    ref = O.fft_ld(x[[0, b - 1]].cpu().numpy().astype(np.complex128))[:, :nout]
     assert O.rel_l2_rows(outs["fused"][[0, b - 1]], ref).max() <= tol
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_real_half_in_place_dense(gpu, prec, monkeypatch):
+    """in == out with the dense layout (n reals in, n/2 + 1 complex out per
+    transform): each transform's bins overlap the next transform's input, so
+    every input must be read before a bin is written, also when the batch
+    exceeds the chunk size."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    n, b = 1 << 16, 6
+    nout = n // 2 + 1
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((b, n)).astype(rdt)
+    buf = torch.zeros(b * nout, dtype=tcdt, device=gpu)
+    xin = torch.view_as_real(buf).reshape(-1)[: b * n].view(b, n)
+    xin.copy_(torch.from_numpy(x).to(gpu))
+    monkeypatch.setenv("GFFT_CHUNK_BYTES", str(2 * (n // 2) * (8 if prec == "f32" else 16)))
+    p = G.Plan(C.GFFT_R2C, n, precision=0 if prec == "f32" else 1, batch=b, half=True)
+    assert "r2c_rows_unpack" in p.describe()
+    p.execute(xin, buf.view(b, nout))
+    y = buf.view(b, nout).cpu().numpy()
+    ref = O.fft_ld(x.astype(np.complex128))[:, :nout]
+    assert O.rel_l2_rows(y, ref).max() <= _tol(prec, n)
+
+
+def test_chunked_plan_rejects_overlap_with_other_distances(gpu, monkeypatch):
+    """A C2C plan whose batch runs in chunks refuses in == out when input and
+    output distances differ (later inputs would be overwritten first), and
+    accepts it when every transform is in place."""
+    import genfft_amd as G
+    from genfft_amd import GfftError
+    from genfft_amd import _capi as C
+    n, b = 1 << 16, 4
+    monkeypatch.setenv("GFFT_CHUNK_BYTES", str(n * 8))
+    buf = torch.randn(b * (n + 2), dtype=torch.complex64, device=gpu)
+    p = G.Plan(C.GFFT_C2C, n, batch=b, in_dist=n, out_dist=n + 2)
+    with pytest.raises(GfftError, match="chunks"):
+        p.execute(buf, buf)
+    q = G.Plan(C.GFFT_C2C, n, batch=b)
+    x = buf[: b * n].clone()
+    ref = O.fft_ld(x.view(b, n).cpu().numpy().astype(np.complex128))
+    q.execute(x, x)
+    assert O.rel_l2_rows(x.view(b, n).cpu().numpy(), ref).max() <= O.tol_c64(n)
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
