@@ -65,6 +65,9 @@ OTHER = {
     "r2c_f32_n2^20_b256": ("r2c", 1 << 20, 0, 256),
     "c2d_c64_4096x4096": ("c2c_2d", 4096, 4096, 1),
     "c2d_c64_8192x8192": ("c2c_2d", 8192, 8192, 1),
+    "c2r_f32_n2^20_b256": ("c2r", 1 << 20, 0, 256),
+    "r2c2d_f32_4096x4096": ("r2c_2d", 4096, 4096, 1),
+    "c2r2d_f32_4096x4096": ("c2r_2d", 4096, 4096, 1),
 }
 
 
@@ -87,7 +90,7 @@ def parse():
 def tile_launches(text):
     out = []
     for ln in text.splitlines():
-        if not ln.startswith(("tile", "r2c_rows_unpack", "unpack", "pack", "hfill", "copy")):
+        if not ln.startswith(("tile", "r2c_rows_unpack", "c2r_pack_cols", "unpack", "pack", "hfill", "copy")):
             continue
         f = dict(kv.split("=", 1) for kv in ln.split()[1:] if "=" in kv)
         f["kind"] = ln.split()[0]
@@ -332,6 +335,21 @@ def main():
         x = torch.complex(torch.randn(n2, n, device=dev, generator=gen), torch.randn(n2, n, device=dev, generator=gen))
         y = torch.empty_like(x)
         plan = G.Plan(C.GFFT_C2C_2D, n, precision=0, n2=n2, batch=batch)
+    elif kind == "c2r":
+        # half spectra of float32 N(0,1) rows (bins 0 and n/2 real) -> n reals each, unscaled
+        x = torch.fft.rfft(torch.randn(batch, n, device=dev, generator=gen))
+        y = torch.empty(batch, n, device=dev)
+        plan = G.Plan(C.GFFT_C2R, n, precision=0, batch=batch)
+    elif kind == "r2c_2d":
+        # real n2 x n image -> the full n2 x n complex spectrum (RealFFT2D::forward)
+        x = torch.randn(n2, n, device=dev, generator=gen)
+        y = torch.empty(n2, n, dtype=torch.complex64, device=dev)
+        plan = G.Plan(C.GFFT_R2C_2D, n, precision=0, n2=n2, batch=batch)
+    elif kind == "c2r_2d":
+        # the n/2 + 1 non-redundant columns of a real image's spectrum -> the image (RealFFT2D::inverse)
+        x = torch.fft.rfft2(torch.randn(n2, n, device=dev, generator=gen))
+        y = torch.empty(n2, n, device=dev)
+        plan = G.Plan(C.GFFT_C2R_2D, n, precision=0, n2=n2, batch=batch, in_stride=n // 2 + 1, out_stride=n)
     else:
         # re/im i.i.d. N(0,1) from a fixed seed
         x = torch.complex(torch.randn(batch, n, device=dev, generator=gen, dtype=cdt.to_real()),
@@ -349,24 +367,34 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    inv = kind in ("c2r", "c2r_2d")
     for _ in range(a.warmup):
-        plan.execute(x, y)
+        plan.execute(x, y, inverse=inv)
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record(s)
     for _ in range(a.steps):
-        plan.execute(x, y)
+        plan.execute(x, y, inverse=inv)
     e1.record(s)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_time(e1)
     ms_step = ms_total / a.steps
 
-    if kind == "r2c":
+    if kind in ("r2c", "c2r"):
+        # real-input convention: half the complex flops; n reals and n/2 + 1 bins
         flops = 2.5 * n * math.log2(n) * batch
         alg_bytes = (4.0 * n + 8.0 * (n // 2 + 1)) * batch
+    elif kind == "r2c_2d":
+        N = n * n2
+        flops = 2.5 * N * math.log2(N) * batch
+        alg_bytes = (4.0 * N + 8.0 * N) * batch
+    elif kind == "c2r_2d":
+        N = n * n2
+        flops = 2.5 * N * math.log2(N) * batch
+        alg_bytes = (8.0 * (n // 2 + 1) * n2 + 4.0 * N) * batch
     elif kind == "c2c_2d":
         N = n * n2
         flops = 5.0 * N * math.log2(N) * batch
@@ -381,7 +409,7 @@ def main():
     reps = 10
     acc = [0.0] * len(launches)
     for _ in range(reps):
-        ms = plan.profile(x, y)
+        ms = plan.profile(x, y, inverse=inv)
         for i, v in enumerate(ms[:len(launches)]):
             acc[i] += v / reps
     groups = {}
@@ -390,6 +418,10 @@ def main():
             key = f"tile L={l['L']} M={l['M']} {'row' if l['inp'] == '0' else 'col'}->" \
                   f"{'row' if l['outp'] == '0' else 'col'}"
             elems = int(l["n_inner"]) * int(l["tiles"]) // int(l["tiles_inner"]) * int(l["L"])
+        elif l["kind"] == "c2r_pack_cols":
+            # reads n/2 + 1 bins, writes the n/2-point column-pass output per transform
+            key = f"c2r pack+cols L={l['L']} M={l['M']}"
+            elems = int(l["tiles"]) // (int(l["n2"]) // int(l["M"])) * int(l["L"]) * int(l["n2"])
         elif l["kind"] == "r2c_rows_unpack":
             # reads the n/2-point column-pass output, writes n/2 + 1 bins per transform
             key = f"r2c rows+unpack R={l['R']} M={l['M']}"
@@ -448,9 +480,13 @@ def main():
         "data": "synthetic: re/im i.i.d. N(0,1), torch generator seed 0, resident in HBM",
         "config": {"workload": (f"batched 1D c2c forward, "
                                 f"{ {'c64': 'complex64', 'c128': 'complex128', 'f16': 'FP16 pairs', 'bf16': 'BF16 pairs'}[prec]}, "
-                                f"N=2^{int(math.log2(n))}, batch {batch}") if kind == "c2c" else
-                               (f"batched real float32 -> n/2+1 bins, N=2^{int(math.log2(n))}, batch {batch}"
-                                if kind == "r2c" else f"2D complex64 forward {n}x{n2}"),
+                                f"N=2^{int(math.log2(n))}, batch {batch}") if kind == "c2c" else {
+                       "r2c": f"batched real float32 -> n/2+1 bins, N=2^{int(math.log2(n))}, batch {batch}",
+                       "c2r": f"batched n/2+1 bins -> real float32 (unscaled inverse), N=2^{int(math.log2(n))}, "
+                              f"batch {batch}",
+                       "c2c_2d": f"2D complex64 forward {n}x{n2}",
+                       "r2c_2d": f"2D real float32 {n}x{n2} -> full complex64 spectrum",
+                       "c2r_2d": f"2D n/2+1 spectrum columns -> real float32 {n}x{n2} (unscaled inverse)"}[kind],
                    "n": n, "batch": batch, "precision": prec, "plan": [k for k in groups]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_key,

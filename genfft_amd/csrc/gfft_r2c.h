@@ -39,4 +39,27 @@ const KernelEntry *r2c_fused_entry(int prec);
 
 hipError_t launch_r2c_fused(int prec, const R2cArgs &a, hipStream_t s);  // one workgroup per tile
 
+// The inverse (c2r) mirror image: the pack Z_k = (F_k + conj F_{L-k})
+// + i W_n^-k (F_k - conj F_{L-k}) fused into the first four-step pass of the
+// L-point inverse, L = n1 * n2.  Element e of column c is Z_{c + n2 e}, whose
+// partner L - k is element n1 - 1 - e of column n2 - c, so a tile of M/2
+// columns and their mirrors packs in LDS, runs the n1-point column FFTs
+// (inverse by re/im swap) and stores with the four-step twiddle W_L^(c k_c).
+struct C2rArgs {
+    const void *in;    // bins 0 .. L of transform b at in + b * in_dist
+    void *out;         // element k_c of column c of transform b at out + b * L + c + n2 * k_c
+    int64_t in_dist;
+    int32_t n2;        // columns (the second pass's length)
+    int32_t tiles_per; // workgroups per transform: n2 / M
+    int32_t n_tiles;
+    int32_t xcd_remap;
+    const void *pass_tw;  // Stockham pass twiddles of the n1-point column FFT
+    TwTable tw;           // W_n, n = 2L (the pack)
+    TwTable tw4;          // W_L (the four-step twiddle)
+};
+
+// Shape of the fused column kernel for column length n1 (nullptr: none).
+const KernelEntry *c2r_fused_entry(int prec, int n1);
+hipError_t launch_c2r_fused(int prec, int n1, const C2rArgs &a, hipStream_t s);
+
 }  // namespace gfft

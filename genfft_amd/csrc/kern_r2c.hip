@@ -184,6 +184,160 @@ hipError_t launch_t(const R2cArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// c2r: the pack fused into the column pass of the inverse four-step.
+// ---------------------------------------------------------------------------
+// Z_k from F_k and F_{L-k} with w = W_n^k (pack_kernel's arithmetic,
+// kern_misc.hip): s + i q, s = f + conj g, q = conj(w) (f - conj g).
+template <typename T>
+__device__ __forceinline__ cx<T> packz(cx<T> f, cx<T> g, cx<double> w)
+{
+    const cx<T> sm = {f.re + g.re, f.im - g.im};
+    const cx<T> d = {f.re - g.re, f.im + g.im};
+    const cx<T> wc = {(T)w.re, (T)(-w.im)};
+    const cx<T> q = cmul(wc, d);
+    return {sm.re - q.im, sm.im + q.re};
+}
+
+// Z_k and Z_{L-k} in place, written re/im-swapped (the inverse runs as the
+// forward transform of the swapped data); W_n^{L-k} = -conj(W_n^k).
+template <typename T, class G>
+__device__ __forceinline__ void pack_pair(cx<T> *lds, int sk, int ek, int sj, int ej, cx<double> w)
+{
+    const cx<T> fk = lds[G::off(sk, ek)], fj = lds[G::off(sj, ej)];
+    const cx<double> wj = {-w.re, w.im};
+    lds[G::off(sk, ek)] = swap_ri(packz(fk, fj, w));
+    lds[G::off(sj, ej)] = swap_ri(packz(fj, fk, wj));
+}
+
+template <typename T, int N1, int M, int NT, int... RS>
+__global__ void __launch_bounds__(NT) c2r_col_pack_kernel(C2rArgs a)
+{
+    constexpr int H = M / 2;
+    constexpr int TOT = M * N1;
+    constexpr int E = TOT / NT;
+    static_assert(TOT % NT == 0 && (H * N1) % NT == 0 && NT % M == 0, "tile must split evenly over threads");
+    using G = TileGeom<N1, M, LAY_ELEM, PAT_COL, PAT_COL>;
+    __shared__ cx<T> lds[G::ELEMS];
+    __shared__ cx<T> nyq;  // F_L (workgroup 0 of a transform)
+    const int tid = threadIdx.x;
+    const int tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int b = tile / a.tiles_per;
+    const int a0 = (tile - b * a.tiles_per) * H;
+    const int n2 = a.n2;
+    const int64_t L = (int64_t)N1 * n2;
+    auto col_of = [&](int s) {
+        if (s < H) return a0 + s;
+        const int m = s - H;
+        return a0 + m == 0 ? n2 / 2 : n2 - a0 - m;
+    };
+
+    // load F along the columns (slot-fast: runs of H bins), all loads first
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(a.in) + (int64_t)b * a.in_dist;
+    cx<T> reg[E];
+#pragma unroll
+    for (int it = 0; it < E; it++) {
+        const int e = tid + it * NT;
+        reg[it] = in[col_of(e % M) + (int64_t)n2 * (e / M)];
+    }
+    if (a0 == 0 && tid == 0) nyq = in[L];
+#pragma unroll
+    for (int it = 0; it < E; it++) {
+        const int e = tid + it * NT;
+        lds[G::off(e % M, e / M)] = reg[it];
+    }
+    __syncthreads();
+
+    // pack: thread takes upper slot s = tid % H at e = tid / H + i * (NT / H)
+    {
+        constexpr int P = H * N1 / NT;
+        const int s = tid % H;
+        const int u = a0 + s;
+        if (u != 0) {
+            const int64_t step = (int64_t)n2 * (NT / H);
+            const int64_t k0 = u + (int64_t)n2 * (tid / H);
+            cx<double> w = tw_w(a.tw, (uint64_t)k0);
+            const cx<double> wr = tw_w(a.tw, (uint64_t)step);
+#pragma unroll
+            for (int i = 0; i < P; i++) {
+                const int e = tid / H + i * (NT / H);
+                pack_pair<T, G>(lds, s, e, H + s, N1 - 1 - e, w);
+                w = cmul(w, wr);
+            }
+        } else {
+            // lane 0 of workgroup 0: columns 0 and n2/2 pair with themselves
+            for (int i = 0; i < P; i++) {
+                const int e = tid / H + i * (NT / H);
+                if (e == 0) {
+                    // Z_0 from F_0 and F_L (their imaginary parts ignored), and
+                    // the self-paired Z_{L/2} (element N1/2 of column 0)
+                    cx<T> f0 = lds[G::off(0, 0)], fl = nyq;
+                    f0.im = (T)0;
+                    fl.im = (T)0;
+                    lds[G::off(0, 0)] = swap_ri(packz(f0, fl, cx<double>{1.0, 0.0}));
+                    const cx<T> fq = lds[G::off(0, N1 / 2)];
+                    lds[G::off(0, N1 / 2)] = swap_ri(packz(fq, fq, tw_w(a.tw, (uint64_t)(L / 2))));
+                } else if (e < N1 / 2) {  // column 0: bins n2*e and n2*(N1-e)
+                    pack_pair<T, G>(lds, 0, e, 0, N1 - e, tw_w(a.tw, (uint64_t)((int64_t)n2 * e)));
+                } else {  // column n2/2 (slot H): bins n2/2 + n2*m and n2/2 + n2*(N1-1-m)
+                    const int m = e - N1 / 2;
+                    pack_pair<T, G>(lds, H, m, H, N1 - 1 - m, tw_w(a.tw, (uint64_t)(n2 / 2 + (int64_t)n2 * m)));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    TileArgs none{};
+    none.tw_mode = TW_NONE;
+    stockham_passes<T, N1, M, NT, G, false, false, 1, RS...>(lds, reinterpret_cast<const cx<T> *>(a.pass_tw), tid,
+                                                            none, 0, 0, Tw4Pre{});
+
+    // store with the four-step twiddle W_L^(c k_c), thread: slot s = tid % M
+    // at k_c = tid / M + i * (NT / M), an FP64 progression per thread
+    cx<T> *out = reinterpret_cast<cx<T> *>(a.out) + (int64_t)b * L;
+    const int s = tid % M;
+    const int c = col_of(s);
+    cx<double> w = tw_w(a.tw4, (uint64_t)c * (uint64_t)(tid / M));
+    const cx<double> wr = tw_w(a.tw4, (uint64_t)c * (uint64_t)(NT / M));
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int kc = tid / M + i * (NT / M);
+        out[c + (int64_t)n2 * kc] = apply_tw(lds[G::off(s, kc)], w);
+        w = cmul(w, wr);
+    }
+}
+
+template <typename T>
+struct C2rShape {
+    KernelEntry entry;
+    void (*fn)(C2rArgs);
+};
+
+// One shape per column length, M = 16 (8 columns and their mirrors), 16
+// elements per thread (8 for n1 = 32, 4 for n1 = 16).
+template <typename T, int N1, int NT, int... RS>
+constexpr C2rShape<T> c2r_shape()
+{
+    return {entry_shape<T, N1, 16, NT, LAY_ELEM, PAT_COL, PAT_COL, RS...>(), &c2r_col_pack_kernel<T, N1, 16, NT, RS...>};
+}
+
+const C2rShape<float> kC2rF32[] = {
+    c2r_shape<float, 32, 64, 8, 4>(),    c2r_shape<float, 64, 64, 8, 8>(),     c2r_shape<float, 128, 128, 16, 8>(),
+    c2r_shape<float, 256, 256, 16, 16>(), c2r_shape<float, 512, 512, 8, 8, 8>(), c2r_shape<float, 1024, 1024, 16, 16, 4>(),
+};
+const C2rShape<double> kC2rF64[] = {
+    c2r_shape<double, 16, 64, 16>(),     c2r_shape<double, 32, 64, 8, 4>(),    c2r_shape<double, 64, 64, 8, 8>(),    c2r_shape<double, 128, 128, 16, 8>(),
+    c2r_shape<double, 256, 256, 16, 16>(), c2r_shape<double, 512, 512, 8, 8, 8>(),
+};
+
+template <typename T, size_t K>
+const C2rShape<T> *c2r_find(const C2rShape<T> (&tab)[K], int n1)
+{
+    for (size_t i = 0; i < K; i++)
+        if (tab[i].entry.L == n1) return &tab[i];
+    return nullptr;
+}
+
 }  // namespace
 
 const KernelEntry *r2c_fused_entry(int prec)
@@ -194,6 +348,30 @@ const KernelEntry *r2c_fused_entry(int prec)
 hipError_t launch_r2c_fused(int prec, const R2cArgs &a, hipStream_t s)
 {
     return prec ? launch_t<double>(a, s) : launch_t<float>(a, s);
+}
+
+const KernelEntry *c2r_fused_entry(int prec, int n1)
+{
+    if (prec) {
+        const auto *e = c2r_find(kC2rF64, n1);
+        return e ? &e->entry : nullptr;
+    }
+    const auto *e = c2r_find(kC2rF32, n1);
+    return e ? &e->entry : nullptr;
+}
+
+hipError_t launch_c2r_fused(int prec, int n1, const C2rArgs &a, hipStream_t s)
+{
+    void (*fn)(C2rArgs) = nullptr;
+    int nt = 0;
+    if (prec) {
+        if (const auto *e = c2r_find(kC2rF64, n1)) fn = e->fn, nt = e->entry.NT;
+    } else {
+        if (const auto *e = c2r_find(kC2rF32, n1)) fn = e->fn, nt = e->entry.NT;
+    }
+    if (!fn) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fn, dim3((unsigned)a.n_tiles), dim3(nt), 0, s, a);
+    return hipGetLastError();
 }
 
 }  // namespace gfft

@@ -403,7 +403,7 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
 // Plan representation
 // ---------------------------------------------------------------------------
 enum BufId { B_IN = 0, B_OUT = 1, B_SCR = 2 };
-enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET, LT_R2C };
+enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET, LT_R2C, LT_C2R };
 
 // bytes of one complex element in memory
 size_t csize_of(int prec, int store) { return store ? 4 : (prec ? 16 : 8); }
@@ -417,6 +417,7 @@ struct Launch {
     CopyArgs ca{};
     MixArgs mix{};
     R2cArgs ra{};
+    C2rArgs cr{};
     int in_buf = B_IN, out_buf = B_OUT;
     int64_t in_off = 0, out_off = 0;  // bytes
     bool vin_ok = false, vout_ok = false;
@@ -751,6 +752,52 @@ struct Builder {
         p->launches.push_back(l);
         return GFFT_OK;
     }
+
+    // Whether the c2r of length n runs as the fused pack-and-column pass plus
+    // the transposing row pass of the n/2-point inverse four-step.
+    bool c2r_fusable(int64_t n) const
+    {
+        const int64_t L = n / 2;
+        if (store || !is_pow2(n) || L <= max_len(prec, PAT_ROW, PAT_ROW)) return false;
+        int l1, l2;
+        four_step_split(prec, ilog2(L), &l1, &l2);
+        const KernelEntry *fe = c2r_fused_entry(prec, 1 << l1);
+        return fe && (1 << l2) % fe->M == 0 && (1 << l2) <= max_len(prec, PAT_ROW, PAT_COL);
+    }
+
+    // c2r of length n (n/2 = n1 * n2): the pack fused into the n1-point
+    // column pass over the whole batch into scratch (every bin is read before
+    // a real is written, so in == out is safe), then the n2-point transposing
+    // row pass writes the reals.
+    int c2r_fused(int64_t n, int64_t batch, int64_t in_dist, int64_t out_dist)
+    {
+        const int64_t L = n / 2;
+        int l1, l2;
+        four_step_split(prec, ilog2(L), &l1, &l2);
+        const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
+        const KernelEntry *fe = c2r_fused_entry(prec, (int)N1);
+        need_scratch(batch * L);
+        Launch l;
+        l.type = LT_C2R;
+        l.k = fe;
+        C2rArgs &c = l.cr;
+        c.in_dist = in_dist;
+        c.n2 = (int32_t)N2;
+        c.tiles_per = (int32_t)(N2 / fe->M);
+        if ((int64_t)c.tiles_per * batch > INT32_MAX) return fail(GFFT_ERR_UNSUPPORTED, "too many tiles");
+        c.n_tiles = (int32_t)(c.tiles_per * batch);
+        int st;
+        if ((st = get_pass_twiddles(p->device, *fe, &c.pass_tw, dry))) return st;
+        if ((st = get_two_level(p->device, n, &c.tw, dry))) return st;
+        if ((st = get_two_level(p->device, L, &c.tw4, dry))) return st;
+        l.in_buf = B_IN;
+        l.out_buf = B_SCR;
+        l.first = true;
+        l.grid = c.n_tiles;
+        p->launches.push_back(l);
+        return tile((int)N2, PAT_ROW, PAT_COL, 0, B_SCR, 0, B_OUT, 0, N1, batch, 1, 0, L, N2, 1, 0, out_dist / 2, 1, N1,
+                    TW_NONE, 0, false, true);
+    }
 };
 
 int64_t dflt(int64_t v, int64_t d) { return v ? v : d; }
@@ -843,6 +890,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             break;
         }
         if (od % 2) return fail(GFFT_ERR_UNSUPPORTED, "C2R output distance must be even");
+        if (b.c2r_fusable(n) && b.opts.r2c_fuse) {
+            st = b.c2r_fused(n, batch, id, od);
+            break;
+        }
         const int64_t L = n / 2;
         b.need_scratch(L * batch);
         st = b.unpack(n, batch, B_IN, id, B_SCR, L, false, true);
@@ -1057,6 +1108,16 @@ int validate(const gfft_plan_s *p)
                 return st;
             break;
         }
+        case LT_C2R: {
+            const C2rArgs &a = l.cr;
+            const int64_t nb = a.n_tiles / a.tiles_per, L = (int64_t)l.k->L * a.n2;
+            if ((int64_t)a.tiles_per * nb != a.n_tiles || (int64_t)a.tiles_per * l.k->M != a.n2)
+                return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d c2r tiles", idx);
+            if ((st = inside(l.in_buf, l.in_off, ((nb - 1) * a.in_dist + L + 1) * cs, "c2r bin load"))) return st;
+            if ((st = inside(l.out_buf, l.out_off, nb * L * cs, "c2r column store"))) return st;
+            if (l.k->lds_bytes > 160 * 1024) return fail(GFFT_ERR_UNSUPPORTED, "internal: LDS");
+            break;
+        }
         case LT_R2C: {
             const R2cArgs &a = l.ra;
             const int64_t nb = a.n_tiles / a.tiles_per, L = (int64_t)a.n1 * l.k->L;
@@ -1126,6 +1187,11 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
                      "half=%d\n", l.type == LT_PACK ? "pack" : "unpack", (long long)l.ua.n, (long long)l.ua.batch,
                      bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off,
                      (long long)l.ua.in_dist, (long long)l.ua.out_dist, l.ua.half);
+            break;
+        case LT_C2R:
+            snprintf(line, sizeof(line), "c2r_pack_cols L=%d M=%d NT=%d n2=%d tiles=%d in=%s+%lld out=%s+%lld "
+                     "in_dist=%lld\n", l.k->L, l.k->M, l.k->NT, l.cr.n2, l.cr.n_tiles, bn[l.in_buf], (long long)l.in_off,
+                     bn[l.out_buf], (long long)l.out_off, (long long)l.cr.in_dist);
             break;
         case LT_R2C:
             snprintf(line, sizeof(line), "r2c_rows_unpack R=%d M=%d NT=%d n1=%d tiles=%d grid=%lld in=%s+%lld out=%s+%lld "
@@ -1226,6 +1292,14 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             HFillArgs a = l.ha;
             a.out = op;
             HIP_TRY(launch_hfill(p->prec, a, s));
+            break;
+        }
+        case LT_C2R: {
+            C2rArgs a = l.cr;
+            a.in = ip;
+            a.out = op;
+            a.xcd_remap = p->xcd_remap;
+            HIP_TRY(launch_c2r_fused(p->prec, l.k->L, a, s));
             break;
         }
         case LT_R2C: {

@@ -277,6 +277,31 @@ def test_r2c_half_fuses_the_unpack_into_the_row_pass(prec, monkeypatch):
     assert st == C.GFFT_OK and "r2c_rows_unpack" not in text and "unpack n=" in text
 
 
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
+def test_c2r_fuses_the_pack_into_the_column_pass(prec, monkeypatch):
+    """C2R of a four-step length n/2 = n1 * n2 with a fused shape for n1:
+    the pack-and-column launch plus the transposing row pass; other lengths
+    keep the separate pack.  Bounds proofs hold for padded distances."""
+    maxrow = 16384 if prec == C.GFFT_F32 else 8192
+    shapes = {16, 32, 64, 128, 256, 512} | ({1024} if prec == C.GFFT_F32 else set())
+    for k in range(1, 24):
+        n = 1 << k
+        L = n // 2
+        for ind, outd, b in ((0, 0, 1), (L + 4, n + 6, 3)):
+            st, text = explain(desc(kind=C.GFFT_C2R, prec=prec, n=n, batch=b, ind=ind, outd=outd))
+            assert st == C.GFFT_OK, (n, C.lib().gfft_last_error())
+            fused = "c2r_pack_cols" in text
+            if fused:
+                assert L > maxrow and len(tiles(text)) == 1 and "pack n=" not in text
+                n1 = int(re.search(r"c2r_pack_cols L=(\d+)", text).group(1))
+                assert n1 in shapes and L % n1 == 0
+            else:
+                assert L <= maxrow or L // 1024 not in shapes or n == 1 << 23, (n, text)
+    assert "c2r_pack_cols" in explain(desc(kind=C.GFFT_C2R, prec=prec, n=1 << 20, batch=2))[1]
+    monkeypatch.setenv("GFFT_R2C_FUSE", "0")
+    assert "c2r_pack_cols" not in explain(desc(kind=C.GFFT_C2R, prec=prec, n=1 << 20, batch=2))[1]
+
+
 def test_4096_column_tile_serves_columns_not_four_step():
     """complex64: 4096-row column transforms (2D 4096^2, FFTVert) run one
     4-column tile pass; the four-step of 2^22 keeps 2048 x 2048 and the fused

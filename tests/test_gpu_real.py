@@ -249,6 +249,68 @@ def test_real_half_in_place_dense(gpu, prec, monkeypatch):
     assert O.rel_l2_rows(y, ref).max() <= _tol(prec, n)
 
 
+def _hermitian_full(F: np.ndarray, n: int) -> np.ndarray:
+    """The n-bin spectrum of a real signal from its n/2 + 1 bins (imaginary
+    parts of bins 0 and n/2 dropped, as any c2r does)."""
+    F = F.copy()
+    F[:, 0] = F[:, 0].real
+    F[:, n // 2] = F[:, n // 2].real
+    return np.concatenate([F, np.conj(F[:, 1:n // 2][:, ::-1])], axis=1)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("lg", range(15, 23))
+@pytest.mark.parametrize("pad", [0, 5])
+def test_c2r_fused_pack_vs_oracle(gpu, prec, lg, pad):
+    """c2r of four-step lengths, where the pack runs inside the first pass
+    (column c and its mirror n2 - c in one workgroup): every sample of every
+    transform against the oracle's inverse DFT of the Hermitian-extended
+    spectrum (non-zero imaginary parts at bins 0 and n/2 included, which a
+    c2r ignores), dense and padded distances."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    n, b = 1 << lg, 3
+    nin = n // 2 + 1
+    ind, outd = nin + pad, n + 2 * pad
+    rng = np.random.default_rng(lg * 3 + pad)
+    F = (rng.standard_normal((b, nin)) + 1j * rng.standard_normal((b, nin))).astype(cdt)
+    src = torch.zeros(b, ind, dtype=tcdt, device=gpu)
+    src[:, :nin] = torch.from_numpy(F).to(gpu)
+    out = torch.full((b, outd), 7.0, dtype=trdt, device=gpu)
+    p = G.Plan(C.GFFT_C2R, n, precision=0 if prec == "f32" else 1, batch=b, in_dist=ind, out_dist=outd)
+    p.execute(src, out, inverse=True)
+    y = out.cpu().numpy()
+    ref = O.fft_ld(_hermitian_full(F.astype(np.complex128), n), inv=True).real
+    assert O.rel_l2_rows(y[:, :n], ref).max() <= _tol(prec, n)
+    assert np.all(y[:, n:] == 7.0), "write between outputs"
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_c2r_fused_in_place_and_matches_unfused(gpu, prec, monkeypatch):
+    """N = 2^20: the fused c2r in place (reals over their own spectrum, dense
+    n/2 + 1 -> n layout) and the separate-pack plan agree to rounding."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rdt, cdt, trdt, tcdt = _dts(prec)
+    n, b = 1 << 20, 4
+    nin = n // 2 + 1
+    g = torch.Generator(device=gpu).manual_seed(11)
+    F = G.rfft(torch.randn(b, n, device=gpu, generator=g, dtype=trdt), half=True)
+    p = G.Plan(C.GFFT_C2R, n, precision=0 if prec == "f32" else 1, batch=b, out_dist=2 * nin)
+    assert "c2r_pack_cols" in p.describe()
+    buf = F.clone()
+    p.execute(buf, buf, inverse=True)  # out_dist 2 * nin reals: transform b's reals over its own bins
+    fused = torch.view_as_real(buf).reshape(b, 2 * nin)[:, :n].cpu().numpy()
+    with monkeypatch.context() as m:
+        m.setenv("GFFT_R2C_FUSE", "0")
+        q = G.Plan(C.GFFT_C2R, n, precision=0 if prec == "f32" else 1, batch=b)
+    assert "c2r_pack_cols" not in q.describe()
+    y = torch.empty(b, n, dtype=trdt, device=gpu)
+    q.execute(F, y, inverse=True)
+    assert O.rel_l2_rows(fused, y.cpu().numpy()).max() <= _tol(prec, n)
+
+
 def test_chunked_plan_rejects_overlap_with_other_distances(gpu, monkeypatch):
     """A C2C plan whose batch runs in chunks refuses in == out when input and
     output distances differ (later inputs would be overwritten first), and
