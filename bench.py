@@ -90,7 +90,8 @@ def parse():
 def tile_launches(text):
     out = []
     for ln in text.splitlines():
-        if not ln.startswith(("tile", "r2c_rows_unpack", "c2r_pack_cols", "unpack", "pack", "hfill", "copy")):
+        if not ln.startswith(("tile", "r2c_rows_unpack", "c2r_pack_cols", "real_rows", "unpack", "pack", "hfill",
+                              "copy")):
             continue
         f = dict(kv.split("=", 1) for kv in ln.split()[1:] if "=" in kv)
         f["kind"] = ln.split()[0]
@@ -418,6 +419,10 @@ def main():
             key = f"tile L={l['L']} M={l['M']} {'row' if l['inp'] == '0' else 'col'}->" \
                   f"{'row' if l['outp'] == '0' else 'col'}"
             elems = int(l["n_inner"]) * int(l["tiles"]) // int(l["tiles_inner"]) * int(l["L"])
+        elif l["kind"] == "real_rows":
+            # rows of n = 2L reals <-> bins in one pass (unpack / pack fused)
+            key = f"real rows {l['mode']} L={l['L']} M={l['M']}"
+            elems = int(l["rows"]) * int(l["L"])
         elif l["kind"] == "c2r_pack_cols":
             # reads n/2 + 1 bins, writes the n/2-point column-pass output per transform
             key = f"c2r pack+cols L={l['L']} M={l['M']}"

@@ -18,6 +18,42 @@
 
 namespace gfft {
 
+// W_n^m from a two-level FP64 table
+__device__ __forceinline__ cx<double> tw_w(const TwTable &t, uint64_t m)
+{
+    m &= t.mask;
+    const cx<double> *lo = reinterpret_cast<const cx<double> *>(t.lo);
+    const cx<double> *hi = reinterpret_cast<const cx<double> *>(t.hi);
+    return cmul(hi[m >> t.lo_bits], lo[m & ((1ull << t.lo_bits) - 1ull)]);
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> cj(cx<T> a) { return {a.re, -a.im}; }
+
+// F_k and F_{L-k} from Z_k, Z_{L-k} (generic/fft_dit_impl_generic.inl:48-52)
+template <typename T>
+__device__ __forceinline__ void unpack_pair(cx<T> zk, cx<T> zj, cx<double> w, cx<T> &fk, cx<T> &fj)
+{
+    const cx<T> e = {(T)0.5 * (zk.re + zj.re), (T)0.5 * (zk.im - zj.im)};
+    const cx<T> o = {(T)0.5 * (zk.re - zj.re), (T)0.5 * (zk.im + zj.im)};
+    const cx<T> t = {(T)(-w.im), (T)w.re};  // i * W_n^k
+    const cx<T> to = cmul(t, o);
+    fk = e - to;
+    fj = cj(e + to);
+}
+
+// Z_k from F_k and F_{L-k} with w = W_n^k (pack_kernel's arithmetic,
+// kern_misc.hip): s + i q, s = f + conj g, q = conj(w) (f - conj g).
+template <typename T>
+__device__ __forceinline__ cx<T> packz(cx<T> f, cx<T> g, cx<double> w)
+{
+    const cx<T> sm = {f.re + g.re, f.im - g.im};
+    const cx<T> d = {f.re - g.re, f.im + g.im};
+    const cx<T> wc = {(T)w.re, (T)(-w.im)};
+    const cx<T> q = cmul(wc, d);
+    return {sm.re - q.im, sm.im + q.re};
+}
+
 struct R2cArgs {
     const void *in;    // row r of transform b at in + b * in_dist + r * R (complex elements)
     void *out;         // bins 0 .. L of transform b at out + b * out_dist
@@ -61,5 +97,21 @@ struct C2rArgs {
 // Shape of the fused column kernel for column length n1 (nullptr: none).
 const KernelEntry *c2r_fused_entry(int prec, int n1);
 hipError_t launch_c2r_fused(int prec, int n1, const C2rArgs &a, hipStream_t s);
+
+// Real transforms whose n/2 points fit one row tile: the r2c is the row
+// FFT of the (even, odd) pairs with the unpack as its epilogue, the c2r the
+// pack as its prologue and the inverse row FFT; one HBM round trip either way.
+struct RealRowsArgs {
+    TileArgs t;      // the rows as a ROW -> ROW tile pass (reals read or written as complex pairs)
+    TwTable tw;      // W_n
+    int32_t half;    // r2c: n/2 + 1 bins per row (else all n, Hermitian-filled)
+    int32_t pad;
+};
+
+enum RealRowsMode { RR_R2C = 0, RR_C2R = 1 };
+
+// Shape for n/2 = L (the row-tile shape of that length; nullptr: none).
+const KernelEntry *real_rows_entry(int prec, int L);
+hipError_t launch_real_rows(int prec, int L, int mode, const RealRowsArgs &a, int64_t grid, hipStream_t s);
 
 }  // namespace gfft

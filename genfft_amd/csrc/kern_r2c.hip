@@ -17,29 +17,6 @@
 namespace gfft {
 namespace {
 
-__device__ __forceinline__ cx<double> tw_w(const TwTable &t, uint64_t m)
-{
-    m &= t.mask;
-    const cx<double> *lo = reinterpret_cast<const cx<double> *>(t.lo);
-    const cx<double> *hi = reinterpret_cast<const cx<double> *>(t.hi);
-    return cmul(hi[m >> t.lo_bits], lo[m & ((1ull << t.lo_bits) - 1ull)]);
-}
-
-template <typename T>
-__device__ __forceinline__ cx<T> cj(cx<T> a) { return {a.re, -a.im}; }
-
-// F_k and F_{L-k} from Z_k, Z_{L-k} (generic/fft_dit_impl_generic.inl:48-52)
-template <typename T>
-__device__ __forceinline__ void unpack_pair(cx<T> zk, cx<T> zj, cx<double> w, cx<T> &fk, cx<T> &fj)
-{
-    const cx<T> e = {(T)0.5 * (zk.re + zj.re), (T)0.5 * (zk.im - zj.im)};
-    const cx<T> o = {(T)0.5 * (zk.re - zj.re), (T)0.5 * (zk.im + zj.im)};
-    const cx<T> t = {(T)(-w.im), (T)w.re};  // i * W_n^k
-    const cx<T> to = cmul(t, o);
-    fk = e - to;
-    fj = cj(e + to);
-}
-
 // Bins of one tile (transform b, upper rows a0 ..) from the row spectra in
 // LDS; thread `t` takes slot s = t % H at k_r = t / H + i * (NT / H).
 template <typename T, int R, int M, int NT, class G>
@@ -187,18 +164,6 @@ hipError_t launch_t(const R2cArgs &a, hipStream_t s)
 // ---------------------------------------------------------------------------
 // c2r: the pack fused into the column pass of the inverse four-step.
 // ---------------------------------------------------------------------------
-// Z_k from F_k and F_{L-k} with w = W_n^k (pack_kernel's arithmetic,
-// kern_misc.hip): s + i q, s = f + conj g, q = conj(w) (f - conj g).
-template <typename T>
-__device__ __forceinline__ cx<T> packz(cx<T> f, cx<T> g, cx<double> w)
-{
-    const cx<T> sm = {f.re + g.re, f.im - g.im};
-    const cx<T> d = {f.re - g.re, f.im + g.im};
-    const cx<T> wc = {(T)w.re, (T)(-w.im)};
-    const cx<T> q = cmul(wc, d);
-    return {sm.re - q.im, sm.im + q.re};
-}
-
 // Z_k and Z_{L-k} in place, written re/im-swapped (the inverse runs as the
 // forward transform of the swapped data); W_n^{L-k} = -conj(W_n^k).
 template <typename T, class G>

@@ -403,7 +403,7 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
 // Plan representation
 // ---------------------------------------------------------------------------
 enum BufId { B_IN = 0, B_OUT = 1, B_SCR = 2 };
-enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET, LT_R2C, LT_C2R };
+enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET, LT_R2C, LT_C2R, LT_RROWS };
 
 // bytes of one complex element in memory
 size_t csize_of(int prec, int store) { return store ? 4 : (prec ? 16 : 8); }
@@ -418,6 +418,8 @@ struct Launch {
     MixArgs mix{};
     R2cArgs ra{};
     C2rArgs cr{};
+    RealRowsArgs rr{};
+    int rr_mode = RR_R2C;
     int in_buf = B_IN, out_buf = B_OUT;
     int64_t in_off = 0, out_off = 0;  // bytes
     bool vin_ok = false, vout_ok = false;
@@ -479,11 +481,15 @@ struct gfft_plan_s {
     int num_cus = 256;
     size_t in_bytes = 0, out_bytes = 0;
     bool out_has_gaps = false;
-    // A chunked plan writes the output of early batch members before it
-    // reads the input of later ones; in_step / out_step are the batch
-    // distances in bytes (1D kinds), see check_alias.
-    bool chunked = false;
-    int64_t in_step = 0, out_step = 0;
+    // Set when a launch may write part of the output before another part of
+    // the input is read (a batch run in chunks, tiles of rows writing bins
+    // where other rows' reals lie).  Overlapping buffers are then staged
+    // (needs_stage) unless every transform is in place: in == out with equal
+    // byte distances in_step / out_step (batch members, or rows) and
+    // in_step2 / out_step2 (2D images).
+    bool overlap_hazard = false;
+    int64_t in_step = 0, out_step = 0, in_step2 = 0, out_step2 = 0;
+    void *stage = nullptr;  // copy of an input that overlaps the output (allocated on first use)
     std::mutex host_mu;
     void *hin = nullptr, *hout = nullptr;
     StreamOrder order;  // used when the plan has a workspace
@@ -628,7 +634,7 @@ struct Builder {
         const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
         int64_t chunk = std::max<int64_t>(1, chunk_bytes / (n * (int64_t)csize));
         chunk = std::min(chunk, batch);
-        if (chunk < batch) p->chunked = true;
+        if (chunk < batch) p->overlap_hazard = true;
         need_scratch(chunk * n);
         const int pm1 = opts.col_m, pm2 = opts.row_m;
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
@@ -753,6 +759,57 @@ struct Builder {
         return GFFT_OK;
     }
 
+    // Whether a real transform of length n runs in one row-tile pass (n/2 a
+    // single-pass row length from 64 on): unpack or pack fused in.
+    bool rows_fusable(int64_t n) const
+    {
+        return !store && opts.r2c_fuse && is_pow2(n) && n / 2 >= 64 && real_rows_entry(prec, (int)(n / 2));
+    }
+
+    // `rows` real transforms of length n in one launch: mode RR_R2C reads
+    // rows of n reals (in_row apart, complex units) and writes bins (out_row
+    // apart; n/2 + 1 of them with `half`, else n); RR_C2R reads n/2 + 1 bins
+    // per row and writes n reals (out_row apart, complex units).
+    int real_rows(int mode, int64_t n, int64_t rows, int in_buf, int64_t in_off_el, int64_t in_row, int out_buf,
+                  int64_t out_off_el, int64_t out_row, bool half)
+    {
+        const KernelEntry *fe = real_rows_entry(prec, (int)(n / 2));
+        Launch l;
+        l.type = LT_RROWS;
+        l.k = fe;
+        l.rr_mode = mode;
+        TileArgs &a = l.rr.t;
+        const int64_t tiles = (rows + fe->M - 1) / fe->M;
+        if (tiles > INT32_MAX || rows > INT32_MAX) return fail(GFFT_ERR_UNSUPPORTED, "too many rows");
+        a.in_inner = in_row;
+        a.in_elem = 1;
+        a.out_inner = out_row;
+        a.out_elem = 1;
+        a.n_inner = (int32_t)rows;
+        a.tiles_inner = (int32_t)tiles;
+        a.n_outer1 = 1;
+        a.n_tiles = (int32_t)tiles;
+        a.tw_mode = TW_NONE;
+        a.swap_out = mode == RR_C2R;
+        l.rr.half = half ? 1 : 0;
+        int st;
+        if ((st = get_pass_twiddles(p->device, *fe, &a.pass_tw, dry))) return st;
+        if ((st = get_two_level(p->device, n, &l.rr.tw, dry))) return st;
+        auto even = [](int64_t v) { return (v & 1) == 0; };
+        if (prec == 0) {
+            l.vin_ok = mode == RR_R2C && even(in_row) && even(in_off_el);
+            l.vout_ok = mode == RR_C2R && even(out_row) && even(out_off_el);
+        }
+        l.in_buf = in_buf;
+        l.out_buf = out_buf;
+        l.in_off = in_off_el * (int64_t)csize;
+        l.out_off = out_off_el * (int64_t)csize;
+        l.grid = tiles;
+        p->launches.push_back(l);
+        p->overlap_hazard = true;
+        return GFFT_OK;
+    }
+
     // Whether the c2r of length n runs as the fused pack-and-column pass plus
     // the transposing row pass of the n/2-point inverse four-step.
     bool c2r_fusable(int64_t n) const
@@ -842,6 +899,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             break;
         }
         if (id % 2) return fail(GFFT_ERR_UNSUPPORTED, "R2C input distance must be even");
+        if (b.rows_fusable(n)) {
+            st = b.real_rows(RR_R2C, n, batch, B_IN, 0, id / 2, B_OUT, 0, od, half);
+            break;
+        }
         // The fused path needs scratch for the whole batch.  The separate
         // unpack with an odd output distance needs the same; with an even
         // one it chunks the transform through at most GFFT_CHUNK_BYTES, so a
@@ -881,6 +942,8 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         p->in_bytes = ((batch - 1) * id + nin) * cs;
         p->out_bytes = ((batch - 1) * od + n) * rs;
         p->out_has_gaps = od > n;
+        p->in_step = id * (int64_t)cs;
+        p->out_step = od * (int64_t)rs;
         p->dir_mask = 2;
         if (n == 1) {
             Launch l;
@@ -890,6 +953,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             break;
         }
         if (od % 2) return fail(GFFT_ERR_UNSUPPORTED, "C2R output distance must be even");
+        if (b.rows_fusable(n)) {
+            st = b.real_rows(RR_C2R, n, batch, B_IN, 0, id, B_OUT, 0, od / 2, false);
+            break;
+        }
         if (b.c2r_fusable(n) && b.opts.r2c_fuse) {
             st = b.c2r_fused(n, batch, id, od);
             break;
@@ -969,6 +1036,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         p->in_bytes = ((batch - 1) * id + (N - 1) * ip + M) * rs;
         p->out_bytes = ((batch - 1) * od + (N - 1) * op + M) * cs;
         p->out_has_gaps = op > M || od > N * op;
+        p->in_step = ip * (int64_t)rs;
+        p->out_step = op * (int64_t)cs;
+        p->in_step2 = id * (int64_t)rs;
+        p->out_step2 = od * (int64_t)cs;
         p->dir_mask = 1;
         if (M == 1) {
             // a single real column: complex column FFT of (x, 0)
@@ -977,7 +1048,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         if ((ip % 2) || (id % 2)) return fail(GFFT_ERR_UNSUPPORTED, "R2C_2D input pitch must be even");
         const int64_t half = M / 2;
         // row r2c (half spectrum, M/2+1 bins) on every row of every matrix
-        if (batch == 1) {
+        if (b.rows_fusable(M)) {
+            for (int64_t m = 0; m < batch && !st; m++)
+                st = b.real_rows(RR_R2C, M, N, B_IN, m * (id / 2), ip / 2, B_OUT, m * od, op, true);
+        } else if (batch == 1) {
             st = b.c2c_1d(half, N, B_IN, 0, ip / 2, B_OUT, 0, op, true, true);
             if (!st) st = b.unpack(M, N, B_OUT, op, B_OUT, op, true, false);
         } else {
@@ -1024,6 +1098,12 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         const bool uniform = od == N * op;
         for (int64_t m = 0; m < (uniform ? 1 : batch) && !st; m++) {
             const int64_t rowsn = uniform ? N * batch : N;
+            if (b.rows_fusable(M)) {
+                // pack + inverse row FFT in one pass, region B -> output
+                st = b.real_rows(RR_C2R, M, rowsn, B_SCR, m * bsz, vcols, B_OUT, m * (od / 2), op / 2, false);
+                packs.push_back(p->launches.size() - 1);
+                continue;
+            }
             st = b.unpack(M, rowsn, B_SCR, vcols, B_OUT, op / 2, false, true);
             if (st) break;
             packs.push_back(p->launches.size() - 1);
@@ -1108,6 +1188,19 @@ int validate(const gfft_plan_s *p)
                 return st;
             break;
         }
+        case LT_RROWS: {
+            const TileArgs &a = l.rr.t;
+            const int64_t L = l.k->L;
+            const int64_t nin = l.rr_mode == RR_R2C ? L : L + 1;
+            const int64_t nout = l.rr_mode == RR_C2R ? L : (l.rr.half ? L + 1 : 2 * L);
+            if ((st = inside(l.in_buf, l.in_off, ((int64_t)(a.n_inner - 1) * a.in_inner + nin) * cs, "real rows load")))
+                return st;
+            if ((st = inside(l.out_buf, l.out_off, ((int64_t)(a.n_inner - 1) * a.out_inner + nout) * cs,
+                             "real rows store")))
+                return st;
+            if (l.k->lds_bytes > 160 * 1024) return fail(GFFT_ERR_UNSUPPORTED, "internal: LDS");
+            break;
+        }
         case LT_C2R: {
             const C2rArgs &a = l.cr;
             const int64_t nb = a.n_tiles / a.tiles_per, L = (int64_t)l.k->L * a.n2;
@@ -1188,6 +1281,12 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
                      bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off,
                      (long long)l.ua.in_dist, (long long)l.ua.out_dist, l.ua.half);
             break;
+        case LT_RROWS:
+            snprintf(line, sizeof(line), "real_rows mode=%s L=%d M=%d NT=%d rows=%d tiles=%d half=%d in=%s+%lld "
+                     "out=%s+%lld in_row=%lld out_row=%lld\n", l.rr_mode == RR_R2C ? "r2c" : "c2r", l.k->L, l.k->M,
+                     l.k->NT, l.rr.t.n_inner, l.rr.t.n_tiles, l.rr.half, bn[l.in_buf], (long long)l.in_off,
+                     bn[l.out_buf], (long long)l.out_off, (long long)l.rr.t.in_inner, (long long)l.rr.t.out_inner);
+            break;
         case LT_C2R:
             snprintf(line, sizeof(line), "c2r_pack_cols L=%d M=%d NT=%d n2=%d tiles=%d in=%s+%lld out=%s+%lld "
                      "in_dist=%lld\n", l.k->L, l.k->M, l.k->NT, l.cr.n2, l.cr.n_tiles, bn[l.in_buf], (long long)l.in_off,
@@ -1231,14 +1330,38 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
                  void *scratch, int direction, hipStream_t s, hipEvent_t *events = nullptr,
                  const float *scale = nullptr, int64_t scale_div = 1);
 
+// Whether this call's buffers overlap in a way the plan's launch order can
+// corrupt (see gfft_plan_s::overlap_hazard).
+bool needs_stage(const gfft_plan_s *p, const void *in, const void *out)
+{
+    if (!p->overlap_hazard) return false;
+    const uintptr_t i0 = (uintptr_t)in, o0 = (uintptr_t)out;
+    if (!(i0 < o0 + p->out_bytes && o0 < i0 + p->in_bytes)) return false;
+    return !(i0 == o0 && p->in_step == p->out_step && p->in_step2 == p->out_step2);
+}
+
 int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
         hipEvent_t *events = nullptr, const float *scale = nullptr)
 {
-    if (!p->scratch)  // no shared workspace: calls on any streams are independent
+    const bool stage = needs_stage(p, in, out);
+    if (!p->scratch && !stage)  // no shared workspace: calls on any streams are independent
         return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events, scale,
                             p->scale_div);
     std::lock_guard<std::mutex> lk(p->order.mu);
+    if (stage && !p->stage) {
+        hipError_t e = hipMalloc(&p->stage, p->in_bytes);
+        if (e != hipSuccess) {
+            p->stage = nullptr;
+            return fail(GFFT_ERR_ALLOC, "staging buffer of %zu bytes for overlapping in/out: %s", p->in_bytes,
+                        hipGetErrorString(e));
+        }
+    }
     HIP_TRY(p->order.before(s));
+    if (stage) {
+        // the whole input first, so no launch reads what another has overwritten
+        HIP_TRY(hipMemcpyAsync(p->stage, in, p->in_bytes, hipMemcpyDeviceToDevice, s));
+        in = p->stage;
+    }
     int st = run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events, scale,
                           p->scale_div);
     if (st) return st;
@@ -1292,6 +1415,16 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             HFillArgs a = l.ha;
             a.out = op;
             HIP_TRY(launch_hfill(p->prec, a, s));
+            break;
+        }
+        case LT_RROWS: {
+            RealRowsArgs a = l.rr;
+            a.t.in = ip;
+            a.t.out = op;
+            a.t.vec_in = l.vin_ok && (((uintptr_t)ip & 15) == 0);
+            a.t.vec_out = l.vout_ok && (((uintptr_t)op & 15) == 0);
+            a.t.xcd_remap = p->xcd_remap;
+            HIP_TRY(launch_real_rows(p->prec, l.k->L, l.rr_mode, a, l.grid, s));
             break;
         }
         case LT_C2R: {
@@ -1866,22 +1999,9 @@ int gfft_plan_destroy(gfft_plan plan)
     if (plan->hin) (void)hipFree(plan->hin);
     if (plan->hout) (void)hipFree(plan->hout);
     if (plan->perm) (void)hipFree(plan->perm);
+    if (plan->stage) (void)hipFree(plan->stage);
     delete plan;
     return GFFT_OK;
-}
-
-// A chunked plan (a batch larger than its workspace) reads the input of
-// later batch members after writing the output of earlier ones, so
-// overlapping buffers are only safe when every transform is in place.
-static int check_alias(const gfft_plan_s *p, const void *in, const void *out)
-{
-    if (!p->chunked) return GFFT_OK;
-    const uintptr_t i0 = (uintptr_t)in, o0 = (uintptr_t)out;
-    const bool overlap = i0 < o0 + p->out_bytes && o0 < i0 + p->in_bytes;
-    if (!overlap || (i0 == o0 && p->in_step == p->out_step)) return GFFT_OK;
-    return fail(GFFT_ERR_UNSUPPORTED,
-                "this plan runs its batch in chunks: overlapping in/out buffers need in == out and equal "
-                "input and output distances in bytes (or a batch within GFFT_CHUNK_BYTES)");
 }
 
 int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void *stream)
@@ -1894,7 +2014,6 @@ int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void 
         return fail(GFFT_ERR_INVALID_ARG, "this plan kind supports only the %s direction",
                     plan->dir_mask == 1 ? "forward" : "inverse");
     if (int st = on_plan_device(plan->device)) return st;
-    if (int st = check_alias(plan, in, out)) return st;
     return run(plan, in, out, direction, (hipStream_t)stream);
 }
 
@@ -1909,7 +2028,6 @@ int gfft_execute_scaled(gfft_plan plan, const void *in, void *out, int direction
     if (scale_mode != GFFT_SCALE_GIVEN && scale_mode != GFFT_SCALE_AUTO)
         return fail(GFFT_ERR_INVALID_ARG, "scale_mode must be GFFT_SCALE_GIVEN or GFFT_SCALE_AUTO");
     if (int st = on_plan_device(plan->device)) return st;
-    if (int st = check_alias(plan, in, out)) return st;
     hipStream_t s = (hipStream_t)stream;
     if (scale_mode == GFFT_SCALE_AUTO) {
         AmaxArgs a{};
@@ -1996,7 +2114,6 @@ int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int direction, 
 {
     if (!plan || !in || !out || !count) return fail(GFFT_ERR_INVALID_ARG, "null argument");
     if (int st = on_plan_device(plan->device)) return st;
-    if (int st = check_alias(plan, in, out)) return st;
     const int n = (int)plan->launches.size();
     *count = n;
     std::vector<hipEvent_t> ev(2 * n);

@@ -126,11 +126,12 @@ GFFT_API int gfft_plan_destroy(gfft_plan plan);
 /* Executes on device-resident buffers, asynchronously on `stream`
  * (a hipStream_t; NULL = the null stream).  in == out is allowed for every
  * kind except GFFT_COLS / GFFT_C2C_2D / GFFT_R2C_2D / GFFT_C2R_2D with different
- * pitches (C2R_2D also needs out != in: its output is real).  A 1D plan whose
- * batch exceeds its workspace (1 GiB, or GFFT_CHUNK_BYTES) runs the batch in
- * chunks and then accepts overlapping buffers only as in == out with equal
- * input and output distances in bytes; other overlaps return
- * GFFT_ERR_UNSUPPORTED.
+ * pitches (C2R_2D also needs out != in: its output is real).  Where a plan's
+ * launches write part of the output before reading all of the input (a 1D
+ * batch beyond the 1 GiB workspace run in chunks, real transforms of rows
+ * that fit one tile) and the buffers overlap other than as in == out with
+ * equal input and output distances in bytes, the input is first copied to a
+ * plan-owned buffer (allocated on first use) and the plan runs from there.
  * direction: GFFT_FORWARD or GFFT_INVERSE (R2C / R2C_2D / DIT are forward
  * only, C2R inverse only).  Calls may share a plan from any threads and
  * streams, as with genFFT's shared plans (FFTLevel.h:76-92): a call on a

@@ -231,7 +231,9 @@ def test_c2r_2d_plans_in_bounds(prec, w, h):
         if (prec == C.GFFT_F64 and w // 2 > 4096 * 2) or w // 2 > (1 << 22):
             continue
         assert st == C.GFFT_OK, (w, h, ins, outs, b, C.lib().gfft_last_error())
-        assert "pack" in text
+        # rows of 128 .. 32768 (complex64) / 16384 (complex128) reals: pack fused into the row pass
+        fused = 128 <= w <= (32768 if prec == C.GFFT_F32 else 16384)
+        assert ("real_rows mode=c2r" in text) == fused and ("pack n=" in text) != fused, (w, text)
 
 
 def test_c2r_2d_rejections():
@@ -251,15 +253,18 @@ def test_r2c_half_fuses_the_unpack_into_the_row_pass(prec, monkeypatch):
     for k in range(1, 24):
         n = 1 << k
         L = n // 2
-        fused = L > maxrow and L // 512 <= 2048
+        rows = 64 <= L <= maxrow                     # one row-tile pass, unpack as its epilogue
+        fused = L > maxrow and L // 512 <= 2048     # four-step, unpack in the last pass
         for od in (0, n // 2 + 2, n // 2 + 7):
             for b in (1, 4):
                 st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=n, batch=b, outd=od, flags=C.GFFT_FLAG_HALF))
                 assert st == C.GFFT_OK, (n, od, b, C.lib().gfft_last_error())
                 assert ("r2c_rows_unpack" in text) == fused, (n, text)
-                assert ("unpack n=" in text) == (not fused and n > 1), (n, text)
+                assert ("real_rows mode=r2c" in text) == rows, (n, text)
+                assert ("unpack n=" in text) == (not fused and not rows and n > 1), (n, text)
         st, text = explain(desc(kind=C.GFFT_R2C, prec=prec, n=n, batch=2))
         assert st == C.GFFT_OK and "r2c_rows_unpack" not in text
+        assert ("real_rows mode=r2c" in text) == rows
     # the fused plan never chunks (every input is read before a bin is
     # written); past the chunk size an even output distance keeps the
     # chunked separate-unpack path, an odd one (full scratch either way) fuses

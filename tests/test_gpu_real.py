@@ -311,24 +311,57 @@ def test_c2r_fused_in_place_and_matches_unfused(gpu, prec, monkeypatch):
     assert O.rel_l2_rows(fused, y.cpu().numpy()).max() <= _tol(prec, n)
 
 
-def test_chunked_plan_rejects_overlap_with_other_distances(gpu, monkeypatch):
-    """A C2C plan whose batch runs in chunks refuses in == out when input and
-    output distances differ (later inputs would be overwritten first), and
-    accepts it when every transform is in place."""
+def test_overlapping_buffers_are_staged(gpu, monkeypatch):
+    """Plans whose launches write output before reading all input (a batch
+    run in chunks; real rows in one tile pass) copy an overlapping input
+    aside first: in == out with other distances gives the right transform,
+    and the in-place aligned layout needs no copy."""
     import genfft_amd as G
-    from genfft_amd import GfftError
     from genfft_amd import _capi as C
+    rng = np.random.default_rng(21)
+    # chunked complex64 batch, in_dist n, out_dist n + 2, same buffer
     n, b = 1 << 16, 4
     monkeypatch.setenv("GFFT_CHUNK_BYTES", str(n * 8))
-    buf = torch.randn(b * (n + 2), dtype=torch.complex64, device=gpu)
+    x = (rng.standard_normal((b, n)) + 1j * rng.standard_normal((b, n))).astype(np.complex64)
+    buf = torch.zeros(b * (n + 2), dtype=torch.complex64, device=gpu)
+    buf[: b * n] = torch.from_numpy(x.reshape(-1)).to(gpu)
     p = G.Plan(C.GFFT_C2C, n, batch=b, in_dist=n, out_dist=n + 2)
-    with pytest.raises(GfftError, match="chunks"):
-        p.execute(buf, buf)
-    q = G.Plan(C.GFFT_C2C, n, batch=b)
-    x = buf[: b * n].clone()
-    ref = O.fft_ld(x.view(b, n).cpu().numpy().astype(np.complex128))
-    q.execute(x, x)
-    assert O.rel_l2_rows(x.view(b, n).cpu().numpy(), ref).max() <= O.tol_c64(n)
+    p.execute(buf, buf)
+    y = buf.cpu().numpy().reshape(b, n + 2)[:, :n]
+    assert O.rel_l2_rows(y, O.fft_ld(x.astype(np.complex128))).max() <= O.tol_c64(n)
+    monkeypatch.delenv("GFFT_CHUNK_BYTES")
+    for prec in ("f32", "f64"):
+        rdt, cdt, trdt, tcdt = _dts(prec)
+        pr = 0 if prec == "f32" else 1
+        # dense in-place r2c of rows that fit one tile: n reals -> n/2 + 1 bins per row
+        n, b = 4096, 37
+        nin = n // 2 + 1
+        xr = rng.standard_normal((b, n)).astype(rdt)
+        buf = torch.zeros(b * nin, dtype=tcdt, device=gpu)
+        torch.view_as_real(buf).reshape(-1)[: b * n].copy_(torch.from_numpy(xr.reshape(-1)).to(gpu))
+        p = G.Plan(C.GFFT_R2C, n, precision=pr, batch=b, half=True)
+        assert "real_rows" in p.describe()
+        p.execute(torch.view_as_real(buf).reshape(-1)[: b * n], buf)
+        assert O.rel_l2_rows(buf.view(b, nin).cpu().numpy(),
+                             O.fft_ld(xr.astype(np.complex128))[:, :nin]).max() <= _tol(prec, n)
+        # and back: dense in-place c2r (n/2 + 1 bins -> n reals per row)
+        F = buf.view(b, nin).cpu().numpy()
+        q = G.Plan(C.GFFT_C2R, n, precision=pr, batch=b)
+        assert "real_rows" in q.describe()
+        q.execute(buf, torch.view_as_real(buf).reshape(-1)[: b * n], inverse=True)
+        back = torch.view_as_real(buf).reshape(-1)[: b * n].view(b, n).cpu().numpy()
+        ref = O.fft_ld(_hermitian_full(F.astype(np.complex128), n), inv=True).real
+        assert O.rel_l2_rows(back, ref).max() <= _tol(prec, n)
+        assert O.rel_l2_rows(back / n, xr).max() <= 2 * _tol(prec, n)
+    # 2D real forward in place with the dense pitches (M reals in, M complex out)
+    w, h = 256, 64
+    img = rng.standard_normal((h, w)).astype(np.float32)
+    buf = torch.zeros(h * w, dtype=torch.complex64, device=gpu)
+    torch.view_as_real(buf).reshape(-1)[: h * w].copy_(torch.from_numpy(img.reshape(-1)).to(gpu))
+    p = G.Plan(C.GFFT_R2C_2D, w, n2=h)
+    p.execute(torch.view_as_real(buf).reshape(-1)[: h * w], buf)
+    ref = O.fft_ld(O.fft_ld(img.astype(np.complex128)).T.copy()).T
+    assert O.rel_l2(buf.view(h, w).cpu().numpy(), ref) <= O.tol_c64(w * h)
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
