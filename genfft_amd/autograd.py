@@ -106,10 +106,18 @@ def spectral_filter(x: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
     return SpectralFilter.apply(x, H)
 
 
+_weights: dict = {}
+
+
 def _edge_weights(bins: int, n: int, dtype, device) -> torch.Tensor:
-    w = torch.full((bins,), 2.0 / n, dtype=dtype, device=device)
-    w[0] = 1.0 / n
-    w[-1] = 1.0 / n
+    """w_k = 2/n, 1/n at bins 0 and n/2; built once per shape (no host copy per call)."""
+    key = (bins, n, dtype, device)
+    w = _weights.get(key)
+    if w is None:
+        w = torch.full((bins,), 2.0 / n, dtype=dtype, device=device)
+        w[0] = 1.0 / n
+        w[-1] = 1.0 / n
+        _weights[key] = w
     return w
 
 

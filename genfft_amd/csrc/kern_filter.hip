@@ -243,7 +243,7 @@ constexpr FilterTable<T> ft()
 const FilterTable<float> kF32[] = {
     ft<float, 32, 64, 8, 4>(),       ft<float, 64, 64, 8, 8>(),        ft<float, 128, 64, 16, 8>(),
     ft<float, 256, 64, 16, 16>(),    ft<float, 512, 64, 8, 8, 8>(),    ft<float, 1024, 64, 16, 16, 4>(),
-    ft<float, 2048, 128, 16, 16, 8>(), ft<float, 4096, 256, 16, 16, 16>(), ft<float, 8192, 512, 16, 16, 8, 4>(),
+    ft<float, 2048, 128, 16, 16, 8>(), ft<float, 4096, 256, 16, 16, 16>(), ft<float, 8192, 1024, 16, 16, 8, 4>(),
 };
 const FilterTable<double> kF64[] = {
     ft<double, 32, 64, 8, 4>(),      ft<double, 64, 64, 8, 8>(),       ft<double, 128, 64, 16, 8>(),
@@ -288,7 +288,9 @@ hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s
     if (which == 2) {  // gH from the per-row partials: one or two reduction passes
         const int64_t bins = a.bins, rows = a.batch;
         const int64_t gx = (bins + 31) / 32;
-        int64_t S = std::min<int64_t>((1024 + gx - 1) / gx, (rows + 63) / 64);
+        // row segments only while the bin columns alone give fewer than ~256
+        // blocks: a second pass (its own launch) then adds the segment sums
+        int64_t S = std::min<int64_t>((256 + gx - 1) / gx, (rows + 63) / 64);
         if (S < 1) S = 1;
         const int64_t seg = (rows + S - 1) / S;
         S = (rows + seg - 1) / seg;
