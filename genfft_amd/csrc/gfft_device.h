@@ -883,8 +883,10 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
     while (tile < a.n_tiles) {
         // The thread index is rebuilt every iteration: otherwise the compiler
         // hoists every tid-derived LDS offset of every pass out of the loop
-        // and keeps them live, which costs ~100 VGPRs.
+        // and keeps them live, which costs ~100 VGPRs.  Its range, which the
+        // asm hides, is restated so index divisions stay unsigned shifts.
         const int t = fresh_tid(wave);
+        __builtin_assume(t >= 0 && t < NT);
         // four-step twiddle bases of this tile (s-fast mapping: s = t % M):
         // issued first, so their latency hides behind the wait for the
         // prefetched tile data that put_tile needs anyway
@@ -1045,6 +1047,7 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
     while (tile < a.n_tiles) {
         // thread index rebuilt per iteration (see tile_fft_pipe_kernel)
         const int t = fresh_tid(wave);
+        __builtin_assume(t >= 0 && t < NT);
         cx<double> hj{}, lj{}, hd{}, ld{}, hr{}, lr{};
         const bool tw = LAYOUT == LAY_ELEM && NT % M == 0 && a.tw_mode != TW_NONE;
         if (tw) {
