@@ -399,6 +399,37 @@ __device__ __forceinline__ bool swap_out_of(const TileArgs &a, int o)
     return a.job_swap_out ? a.job_inv[a.job_first + o] != 0 : a.swap_out != 0;
 }
 
+// The 16-byte accesses of a tile: access `it` of thread `tid` covers
+// elements e, e + 1 with e = 2 tid + 2 NT it, i.e. element n of sequence s
+// (ROW: s = e / L, n = e % L; COL: n = e / M, s = e % M).  With L, M and NT
+// powers of two its offset splits into a part that depends on tid alone and
+// a part that depends on `it` alone, so every access is one uniform 64-bit
+// base (scalar arithmetic) plus the same 32-bit per-thread byte offset: no
+// per-access 64-bit multiply.  The host enables the vector path only where
+// every in-tile byte offset fits 31 bits.
+template <int L, int M, int NT, int PAT>
+struct VecWalk {
+    static_assert((L & (L - 1)) == 0 && (M & (M - 1)) == 0 && (NT & (NT - 1)) == 0, "powers of two");
+    // element offset of (s, n) = s * inner + n (ROW) or n * elem + s (COL)
+    static __device__ __forceinline__ uint32_t thread_off(int tid, uint32_t inner, uint32_t elem)
+    {
+        const uint32_t e = 2u * (uint32_t)tid;
+        if constexpr (PAT == PAT_ROW) {
+            if constexpr ((2 * NT) % L == 0) return (e / L) * inner + e % L;
+            else return e;  // L is a multiple of 2 NT: a thread stays inside one sequence
+        } else {
+            if constexpr ((2 * NT) % M == 0) return (e / M) * elem + e % M;
+            else return e;  // M is a multiple of 2 NT: a thread stays inside one element row
+        }
+    }
+    static __device__ __forceinline__ int64_t iter_off(int it, int64_t inner, int64_t elem)
+    {
+        const int e = 2 * NT * it;
+        if constexpr (PAT == PAT_ROW) return (int64_t)(e / L) * inner + e % L;
+        else return (int64_t)(e / M) * elem + e % M;
+    }
+};
+
 template <typename T, int L, int M, int NT, class G, int INP>
 __device__ __forceinline__ void load_tile(cx<T> *lds, const TileArgs &a, int o2, int o, int s0,
                                           int nvalid, int tid)
@@ -411,27 +442,21 @@ __device__ __forceinline__ void load_tile(cx<T> *lds, const TileArgs &a, int o2,
 
     if constexpr (sizeof(T) == 4 && TOT % (2 * NT) == 0 && (INP == PAT_ROW ? L : M) % 2 == 0) {
         if (a.vec_in && nvalid == M) {
+            using VW = VecWalk<L, M, NT, INP>;
+            const char *b = reinterpret_cast<const char *>(in);
+            const uint32_t to = VW::thread_off(tid, (uint32_t)a.in_inner, (uint32_t)a.in_elem) * sizeof(cx<T>);
 #pragma unroll
             for (int it = 0; it < TOT / (2 * NT); it++) {
                 const int e = (tid + it * NT) * 2;
-                int s, n;
-                const float4 *p;
+                const float4 q = *reinterpret_cast<const float4 *>(
+                    b + VW::iter_off(it, a.in_inner, a.in_elem) * (int64_t)sizeof(cx<T>) + to);
+                const cx<T> x0 = {q.x, q.y}, x1 = {q.z, q.w};
                 if constexpr (INP == PAT_ROW) {
-                    s = e / L;
-                    n = e % L;
-                    p = reinterpret_cast<const float4 *>(in + (int64_t)s * a.in_inner + n);
-                    float4 q = *p;
-                    cx<T> x0 = {q.x, q.y}, x1 = {q.z, q.w};
-                    lds[G::off(s, n)] = sw ? swap_ri(x0) : x0;
-                    lds[G::off(s, n + 1)] = sw ? swap_ri(x1) : x1;
+                    lds[G::off(e / L, e % L)] = sw ? swap_ri(x0) : x0;
+                    lds[G::off(e / L, e % L + 1)] = sw ? swap_ri(x1) : x1;
                 } else {
-                    n = e / M;
-                    s = e % M;
-                    p = reinterpret_cast<const float4 *>(in + (int64_t)n * a.in_elem + s);
-                    float4 q = *p;
-                    cx<T> x0 = {q.x, q.y}, x1 = {q.z, q.w};
-                    lds[G::off(s, n)] = sw ? swap_ri(x0) : x0;
-                    lds[G::off(s + 1, n)] = sw ? swap_ri(x1) : x1;
+                    lds[G::off(e % M, e / M)] = sw ? swap_ri(x0) : x0;
+                    lds[G::off(e % M + 1, e / M)] = sw ? swap_ri(x1) : x1;
                 }
             }
             return;
@@ -472,25 +497,21 @@ __device__ __forceinline__ void store_tile(const cx<T> *lds, const TileArgs &a, 
 
     if constexpr (sizeof(T) == 4 && TOT % (2 * NT) == 0 && (OUTP == PAT_ROW ? L : M) % 2 == 0) {
         if (a.vec_out && nvalid == M) {
+            using VW = VecWalk<L, M, NT, OUTP>;
+            char *b = reinterpret_cast<char *>(out);
+            const uint32_t to = VW::thread_off(tid, (uint32_t)a.out_inner, (uint32_t)a.out_elem) * sizeof(cx<T>);
 #pragma unroll
             for (int it = 0; it < TOT / (2 * NT); it++) {
                 const int e = (tid + it * NT) * 2;
-                int s, k;
                 cx<T> x0, x1;
-                float4 *p;
                 if constexpr (OUTP == PAT_ROW) {
-                    s = e / L;
-                    k = e % L;
-                    x0 = lds[G::off(s, k)];
-                    x1 = lds[G::off(s, k + 1)];
-                    p = reinterpret_cast<float4 *>(out + (int64_t)s * a.out_inner + k);
+                    x0 = lds[G::off(e / L, e % L)];
+                    x1 = lds[G::off(e / L, e % L + 1)];
                 } else {
-                    k = e / M;
-                    s = e % M;
-                    x0 = lds[G::off(s, k)];
-                    x1 = lds[G::off(s + 1, k)];
-                    p = reinterpret_cast<float4 *>(out + (int64_t)k * a.out_elem + s);
+                    x0 = lds[G::off(e % M, e / M)];
+                    x1 = lds[G::off(e % M + 1, e / M)];
                 }
+                float4 *p = reinterpret_cast<float4 *>(b + VW::iter_off(it, a.out_inner, a.out_elem) * (int64_t)sizeof(cx<T>) + to);
                 if (sw) {
                     x0 = swap_ri(x0);
                     x1 = swap_ri(x1);
@@ -791,13 +812,13 @@ __device__ __forceinline__ bool fetch_tile(cx<T> (&reg)[M * L / NT], const TileA
     else in += s0;
     if constexpr (sizeof(T) == 4 && TOT % (2 * NT) == 0 && (INP == PAT_ROW ? L : M) % 2 == 0) {
         if (a.vec_in && nvalid == M) {
+            using VW = VecWalk<L, M, NT, INP>;
+            const char *b = reinterpret_cast<const char *>(in);
+            const uint32_t to = VW::thread_off(tid, (uint32_t)a.in_inner, (uint32_t)a.in_elem) * sizeof(cx<T>);
 #pragma unroll
             for (int it = 0; it < TOT / (2 * NT); it++) {
-                const int e = (tid + it * NT) * 2;
-                const float4 *p;
-                if constexpr (INP == PAT_ROW) p = reinterpret_cast<const float4 *>(in + (int64_t)(e / L) * a.in_inner + e % L);
-                else p = reinterpret_cast<const float4 *>(in + (int64_t)(e / M) * a.in_elem + e % M);
-                const float4 q = *p;
+                const float4 q = *reinterpret_cast<const float4 *>(
+                    b + VW::iter_off(it, a.in_inner, a.in_elem) * (int64_t)sizeof(cx<T>) + to);
                 reg[2 * it] = {q.x, q.y};
                 reg[2 * it + 1] = {q.z, q.w};
             }

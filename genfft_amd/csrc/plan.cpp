@@ -499,6 +499,17 @@ struct gfft_plan_s {
 namespace gfft {
 namespace {
 
+// The 16-byte paths of the complex64 tile kernels address an access as a
+// uniform base plus one 32-bit per-thread byte offset, so every in-tile
+// offset must fit 31 bits (element n of sequence s lies s * inner + n (ROW)
+// or n * elem + s (COL) from the tile's first element).
+bool tile_offsets_fit(int pat, int64_t L, int64_t M, int64_t inner, int64_t elem, size_t csize)
+{
+    if (inner < 0 || elem < 0) return false;
+    const int64_t last = pat == PAT_ROW ? (M - 1) * inner + (L - 1) : (L - 1) * elem + (M - 1);
+    return last * (int64_t)csize < ((int64_t)1 << 31);
+}
+
 struct Builder {
     gfft_plan_s *p;
     int prec;
@@ -560,9 +571,11 @@ struct Builder {
         auto even = [](int64_t v) { return (v & 1) == 0; };
         if (prec == 0 && store == 0) {
             l.vin_ok = even(in_outer2) && even(in_outer) && even(in_off_el) &&
-                       (inp == PAT_ROW ? even(in_inner) : even(in_elem));
+                       (inp == PAT_ROW ? even(in_inner) : even(in_elem)) &&
+                       tile_offsets_fit(inp, k->L, k->M, in_inner, in_elem, csize);
             l.vout_ok = even(out_outer2) && even(out_outer) && even(out_off_el) &&
-                        (outp == PAT_ROW ? even(out_inner) : even(out_elem));
+                        (outp == PAT_ROW ? even(out_inner) : even(out_elem)) &&
+                        tile_offsets_fit(outp, k->L, k->M, out_inner, out_elem, csize);
         } else if (store) {
             // 16-bit pairs: a 16-byte vector is four elements
             auto q4 = [](int64_t v) { return (v & 3) == 0; };
@@ -797,8 +810,10 @@ struct Builder {
         if ((st = get_two_level(p->device, n, &l.rr.tw, dry))) return st;
         auto even = [](int64_t v) { return (v & 1) == 0; };
         if (prec == 0) {
-            l.vin_ok = mode == RR_R2C && even(in_row) && even(in_off_el);
-            l.vout_ok = mode == RR_C2R && even(out_row) && even(out_off_el);
+            l.vin_ok = mode == RR_R2C && even(in_row) && even(in_off_el) &&
+                       tile_offsets_fit(PAT_ROW, fe->L, fe->M, in_row, 1, csize);
+            l.vout_ok = mode == RR_C2R && even(out_row) && even(out_off_el) &&
+                        tile_offsets_fit(PAT_ROW, fe->L, fe->M, out_row, 1, csize);
         }
         l.in_buf = in_buf;
         l.out_buf = out_buf;
@@ -1263,7 +1278,7 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
             snprintf(line, sizeof(line),
                      "tile L=%d M=%d NT=%d layout=%d inp=%d outp=%d lds=%d pipe=%d grid=%lld rad=%d,%d,%d,%d tiles=%d "
                      "n_inner=%d tiles_inner=%d n_outer1=%d in=%s+%lld out=%s+%lld "
-                     "in_strides=%lld,%lld,%lld,%lld out_strides=%lld,%lld,%lld,%lld tw=%d first=%d last=%d\n",
+                     "in_strides=%lld,%lld,%lld,%lld out_strides=%lld,%lld,%lld,%lld tw=%d first=%d last=%d vin=%d vout=%d\n",
                      l.k->L, l.k->M, l.k->NT, l.k->layout, l.k->inp, l.k->outp, l.k->lds_bytes, l.k->persistent,
                      (long long)l.grid, l.k->rad[0],
                      l.k->nrad > 1 ? l.k->rad[1] : 0, l.k->nrad > 2 ? l.k->rad[2] : 0,
@@ -1271,7 +1286,7 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
                      bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off,
                      (long long)a.in_outer2, (long long)a.in_outer, (long long)a.in_inner, (long long)a.in_elem,
                      (long long)a.out_outer2, (long long)a.out_outer, (long long)a.out_inner,
-                     (long long)a.out_elem, a.tw_mode, (int)l.first, (int)l.last);
+                     (long long)a.out_elem, a.tw_mode, (int)l.first, (int)l.last, (int)l.vin_ok, (int)l.vout_ok);
             break;
         }
         case LT_UNPACK:

@@ -135,6 +135,19 @@ def test_cols_plans_in_bounds(prec, n, cols):
         assert st == C.GFFT_OK, C.lib().gfft_last_error()
 
 
+def test_vector_paths_need_31_bit_tile_offsets():
+    """complex64 tiles move 16-byte vectors addressed as a uniform base plus a
+    32-bit per-thread offset: a column tile whose last element lies 2^31
+    bytes or more from its first falls back to element accesses."""
+    for stride, vec in ((1 << 16, "1"), ((1 << 19) + 2, "0")):
+        st, text = explain(desc(kind=C.GFFT_COLS, n=1024, n2=64, ins=stride, outs=64))
+        assert st == C.GFFT_OK, C.lib().gfft_last_error()
+        t = tiles(text)
+        assert len(t) == 1 and field(t[0], "inp") == "1"
+        assert (int(field(t[0], "L")) - 1) * stride * 8 >= (1 << 31) or vec == "1"
+        assert field(t[0], "vin") == vec and field(t[0], "vout") == "1"
+
+
 @pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64])
 @pytest.mark.parametrize("w,h", [(1, 1), (2, 1), (1, 8), (32, 16), (8, 64), (4096, 4096), (8192, 8192),
                                  (16, 4096), (4096, 2)])
