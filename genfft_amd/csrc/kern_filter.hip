@@ -146,8 +146,14 @@ template <typename T, int L, int NT, int... RS>
 __global__ void __launch_bounds__(NT) filter_bwd_kernel(FilterArgs a)
 {
     using G = TileGeom<L, 1, LAY_SEQ, PAT_ROW, PAT_ROW>;
-    __shared__ cx<T> gl[G::ELEMS];  // rfft(gy), then the gx pass
-    __shared__ cx<T> xl[G::ELEMS];  // rfft(x)
+    // gy and x are the two sequences of one tile, so their forward FFTs run
+    // as one two-sequence pass chain: half the barriers, and the radix-16
+    // passes of an 8192-point row keep all 1024 threads busy.  Sequence 0
+    // has the same offsets in both geometries.
+    using G2 = TileGeom<L, 2, LAY_SEQ, PAT_ROW, PAT_ROW>;
+    __shared__ cx<T> lds[G2::ELEMS];
+    cx<T> *gl = lds;           // rfft(gy), then the gx pass
+    cx<T> *xl = lds + G2::SS;  // rfft(x)
     const int tid = threadIdx.x;
     const int64_t row = blockIdx.x;
     const cx<T> *gy = reinterpret_cast<const cx<T> *>(a.gy) + row * L;
@@ -161,8 +167,7 @@ __global__ void __launch_bounds__(NT) filter_bwd_kernel(FilterArgs a)
     load_row<T, L, NT, G>(gl, gy, a.vec, tid);
     load_row<T, L, NT, G>(xl, x, a.vec, tid);
     __syncthreads();
-    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(gl, ptw, tid, none, 0, 0, Tw4Pre{});
-    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(xl, ptw, tid, none, 0, 0, Tw4Pre{});
+    stockham_passes<T, L, 2, NT, G2, true, false, 1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
     const T inv_n = (T)1 / (T)(2 * L);
     for (int p = tid; p <= L / 2; p += NT) {
         const cx<T> w = tw[p];
