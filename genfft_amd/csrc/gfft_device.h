@@ -199,6 +199,7 @@ struct TileArgs {
     int32_t swap_in, swap_out;
     int32_t vec_in, vec_out;  // 16-byte vector access allowed (alignment proven on host)
     int32_t xcd_remap;    // map blocks so consecutive tiles run on one XCD
+    int32_t runs;         // persistent kernels: each workgroup takes one contiguous run of tiles
     int32_t tw_mode;      // TW_NONE / TW_INNER (c * k) / TW_OUTER (o1 * k)
     uint32_t tw_mask;     // Ntw - 1
     int32_t tw_lo_bits;   // two-level split of the exponent
@@ -894,14 +895,24 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
         nv = min(M, a.n_inner - s0);
     };
     cx<T> reg[E];
-    int tile = slot, o2 = 0, o = 0, s0 = 0, nvalid = 0;
+    // Tile order.  With a.runs each workgroup takes one contiguous run of
+    // tiles, so the tile it prefetches is the neighbour (adjacent sequences of
+    // the same transform) of the one it stores and both touch the same HBM
+    // pages: complex64 / complex128 column passes (128-byte row segments)
+    // run ~5-10% faster.  Otherwise consecutive tiles go to different
+    // workgroups and run at the same time, which suits the transposing row
+    // pass, whose neighbours then fill whole output rows together.
+    int tile = a.runs ? (int)((int64_t)slot * a.n_tiles / grid) : slot;
+    const int t_end = a.runs ? (int)((int64_t)(slot + 1) * a.n_tiles / grid) : a.n_tiles;
+    const int step = a.runs ? 1 : grid;
+    int o2 = 0, o = 0, s0 = 0, nvalid = 0;
     bool vec = false;
-    if (tile < a.n_tiles) {
+    if (tile < t_end) {
         decode(tile, o2, o, s0, nvalid);
         vec = fetch_tile<T, L, M, NT, G, INP>(reg, a, o2, o, s0, nvalid, tid);
     }
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    while (tile < a.n_tiles) {
+    while (tile < t_end) {
         // The thread index is rebuilt every iteration: otherwise the compiler
         // hoists every tid-derived LDS offset of every pass out of the loop
         // and keeps them live, which costs ~100 VGPRs.  Its range, which the
@@ -937,9 +948,9 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
         }
         __syncthreads();
         // prefetch the next tile
-        const int next = tile + grid;
+        const int next = tile + step;
         int no2 = 0, no = 0, ns0 = 0, nnv = 0;
-        if (next < a.n_tiles) {
+        if (next < t_end) {
             decode(next, no2, no, ns0, nnv);
             vec = fetch_tile<T, L, M, NT, G, INP>(reg, a, no2, no, ns0, nnv, t);
         }
@@ -1058,14 +1069,24 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
         nv = min(M, a.n_inner - s0);
     };
     uint32_t reg[E];
-    int tile = slot, o2 = 0, o = 0, s0 = 0, nvalid = 0;
+    // Tile order.  With a.runs each workgroup takes one contiguous run of
+    // tiles, so the tile it prefetches is the neighbour (adjacent sequences of
+    // the same transform) of the one it stores and both touch the same HBM
+    // pages: complex64 / complex128 column passes (128-byte row segments)
+    // run ~5-10% faster.  Otherwise consecutive tiles go to different
+    // workgroups and run at the same time, which suits the transposing row
+    // pass, whose neighbours then fill whole output rows together.
+    int tile = a.runs ? (int)((int64_t)slot * a.n_tiles / grid) : slot;
+    const int t_end = a.runs ? (int)((int64_t)(slot + 1) * a.n_tiles / grid) : a.n_tiles;
+    const int step = a.runs ? 1 : grid;
+    int o2 = 0, o = 0, s0 = 0, nvalid = 0;
     bool vec = false;
-    if (tile < a.n_tiles) {
+    if (tile < t_end) {
         decode(tile, o2, o, s0, nvalid);
         vec = fetch_tile_h<S, L, M, NT, G, INP>(reg, a, o2, o, s0, nvalid, tid);
     }
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    while (tile < a.n_tiles) {
+    while (tile < t_end) {
         // thread index rebuilt per iteration (see tile_fft_pipe_kernel)
         const int t = fresh_tid(wave);
         __builtin_assume(t >= 0 && t < NT);
@@ -1101,9 +1122,9 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
             pre.hr = cmul(hr, lr);
         }
         __syncthreads();
-        const int next = tile + grid;
+        const int next = tile + step;
         int no2 = 0, no = 0, ns0 = 0, nnv = 0;
-        if (next < a.n_tiles) {
+        if (next < t_end) {
             decode(next, no2, no, ns0, nnv);
             vec = fetch_tile_h<S, L, M, NT, G, INP>(reg, a, no2, no, ns0, nnv, t);
         }

@@ -597,6 +597,9 @@ struct Builder {
                             hipSuccess)
                 per_cu = 1;
             l.grid = std::min<int64_t>(ntiles, (int64_t)std::max(1, per_cu) * std::max(1, p->num_cus));
+            // contiguous runs of tiles per workgroup for the column passes of
+            // 8- and 16-byte elements (measured faster); interleaved otherwise
+            a.runs = inp == PAT_COL && csize >= 8;
         }
         p->launches.push_back(l);
         return GFFT_OK;
@@ -1278,7 +1281,7 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
             snprintf(line, sizeof(line),
                      "tile L=%d M=%d NT=%d layout=%d inp=%d outp=%d lds=%d pipe=%d grid=%lld rad=%d,%d,%d,%d tiles=%d "
                      "n_inner=%d tiles_inner=%d n_outer1=%d in=%s+%lld out=%s+%lld "
-                     "in_strides=%lld,%lld,%lld,%lld out_strides=%lld,%lld,%lld,%lld tw=%d first=%d last=%d vin=%d vout=%d\n",
+                     "in_strides=%lld,%lld,%lld,%lld out_strides=%lld,%lld,%lld,%lld tw=%d first=%d last=%d vin=%d vout=%d runs=%d\n",
                      l.k->L, l.k->M, l.k->NT, l.k->layout, l.k->inp, l.k->outp, l.k->lds_bytes, l.k->persistent,
                      (long long)l.grid, l.k->rad[0],
                      l.k->nrad > 1 ? l.k->rad[1] : 0, l.k->nrad > 2 ? l.k->rad[2] : 0,
@@ -1286,7 +1289,7 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
                      bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off,
                      (long long)a.in_outer2, (long long)a.in_outer, (long long)a.in_inner, (long long)a.in_elem,
                      (long long)a.out_outer2, (long long)a.out_outer, (long long)a.out_inner,
-                     (long long)a.out_elem, a.tw_mode, (int)l.first, (int)l.last, (int)l.vin_ok, (int)l.vout_ok);
+                     (long long)a.out_elem, a.tw_mode, (int)l.first, (int)l.last, (int)l.vin_ok, (int)l.vout_ok, a.runs);
             break;
         }
         case LT_UNPACK:

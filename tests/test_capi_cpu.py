@@ -40,6 +40,37 @@ def test_only_the_c_abi_is_exported():
     assert not [f for f in funcs if not f.startswith("gfft_") and not f.startswith("_")]
 
 
+@pytest.mark.parametrize("prec", [C.GFFT_F32, C.GFFT_F64, C.GFFT_F16, C.GFFT_BF16])
+def test_column_passes_take_contiguous_tile_runs(prec):
+    """Persistent column passes of 8- and 16-byte elements give each workgroup
+    one contiguous run of tiles; transposing row passes and 16-bit storage
+    interleave tiles across workgroups."""
+    store = prec in (C.GFFT_F16, C.GFFT_BF16)
+    st, text = explain(desc(prec=prec, n=1 << 20, batch=64))
+    assert st == C.GFFT_OK, C.lib().gfft_last_error()
+    t = tiles(text)
+    assert [field(x, "inp") for x in t] == ["1", "0"]
+    for x in t:
+        want = "1" if field(x, "inp") == "1" and not store and field(x, "pipe") == "1" else "0"
+        assert field(x, "runs") == want, x
+
+
+def test_pattern_copy_probe_loads_and_rejects_bad_shapes():
+    """The benchmark's tile-pattern copy probe is a library of its own (not
+    part of the C ABI); shapes it does not cover fail before any device call."""
+    import os
+    path = os.path.join(os.path.dirname(C.LIB_PATH), "libgfft_pattern_copy.so")
+    lib = ctypes.CDLL(path)
+    fn = lib.gfft_pattern_copy
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    fn.restype = ctypes.c_int
+    for n1, n2, m, mode in ((1024, 1024, 16, 3), (1000, 1024, 16, 0), (1024, 1024, 8, 0),
+                            (1024, 1024, 32, 1), (1024, 1024, 1, 0)):
+        assert fn(None, None, 128, n1, n2, m, mode, None) == -1
+    assert fn(16, 32, 128, 1024, 1024, 16, 0, None) == -1  # pointers not 16-byte aligned
+
+
 def desc(kind=C.GFFT_C2C, prec=C.GFFT_F32, n=1024, n2=0, batch=1, ins=0, outs=0, ind=0, outd=0, flags=0):
     return C.GfftDesc(kind, prec, n, n2, batch, ins, outs, ind, outd, flags, 0)
 
