@@ -199,7 +199,10 @@ struct TileArgs {
     int32_t swap_in, swap_out;
     int32_t vec_in, vec_out;  // 16-byte vector access allowed (alignment proven on host)
     int32_t xcd_remap;    // map blocks so consecutive tiles run on one XCD
-    int32_t runs;         // persistent kernels: each workgroup takes one contiguous run of tiles
+    // persistent kernels' tile order: 0 interleaved (tile, tile + grid, ...),
+    // -1 one contiguous run per workgroup, K > 1 (a power of two) runs of K
+    // consecutive tiles dealt out to the workgroups in turn
+    int32_t runs;
     int32_t tw_mode;      // TW_NONE / TW_INNER (c * k) / TW_OUTER (o1 * k)
     uint32_t tw_mask;     // Ntw - 1
     int32_t tw_lo_bits;   // two-level split of the exponent
@@ -895,16 +898,17 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
         nv = min(M, a.n_inner - s0);
     };
     cx<T> reg[E];
-    // Tile order.  With a.runs each workgroup takes one contiguous run of
-    // tiles, so the tile it prefetches is the neighbour (adjacent sequences of
-    // the same transform) of the one it stores and both touch the same HBM
-    // pages: complex64 / complex128 column passes (128-byte row segments)
-    // run ~5-10% faster.  Otherwise consecutive tiles go to different
-    // workgroups and run at the same time, which suits the transposing row
-    // pass, whose neighbours then fill whole output rows together.
-    int tile = a.runs ? (int)((int64_t)slot * a.n_tiles / grid) : slot;
-    const int t_end = a.runs ? (int)((int64_t)(slot + 1) * a.n_tiles / grid) : a.n_tiles;
-    const int step = a.runs ? 1 : grid;
+    // Tile order.  In runs of consecutive tiles, the tile a workgroup
+    // prefetches is the neighbour (adjacent sequences of the same transform)
+    // of the one it stores and both touch the same HBM pages: complex64 /
+    // complex128 column passes (128-byte row segments) run ~5-10% faster.
+    // Interleaved, consecutive tiles go to different workgroups and run at
+    // the same time, which suits the transposing row pass, whose neighbours
+    // then fill whole output rows together.
+    const int K = a.runs > 0 ? a.runs : 1;  // run length of the dealt-out order
+    int tile = a.runs < 0 ? (int)((int64_t)slot * a.n_tiles / grid) : slot * K;
+    const int t_end = a.runs < 0 ? (int)((int64_t)(slot + 1) * a.n_tiles / grid) : a.n_tiles;
+    auto advance = [&](int t) { return a.runs < 0 || ((t + 1) & (K - 1)) ? t + 1 : t + 1 + (grid - 1) * K; };
     int o2 = 0, o = 0, s0 = 0, nvalid = 0;
     bool vec = false;
     if (tile < t_end) {
@@ -948,7 +952,7 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
         }
         __syncthreads();
         // prefetch the next tile
-        const int next = tile + step;
+        const int next = advance(tile);
         int no2 = 0, no = 0, ns0 = 0, nnv = 0;
         if (next < t_end) {
             decode(next, no2, no, ns0, nnv);
@@ -1069,16 +1073,17 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
         nv = min(M, a.n_inner - s0);
     };
     uint32_t reg[E];
-    // Tile order.  With a.runs each workgroup takes one contiguous run of
-    // tiles, so the tile it prefetches is the neighbour (adjacent sequences of
-    // the same transform) of the one it stores and both touch the same HBM
-    // pages: complex64 / complex128 column passes (128-byte row segments)
-    // run ~5-10% faster.  Otherwise consecutive tiles go to different
-    // workgroups and run at the same time, which suits the transposing row
-    // pass, whose neighbours then fill whole output rows together.
-    int tile = a.runs ? (int)((int64_t)slot * a.n_tiles / grid) : slot;
-    const int t_end = a.runs ? (int)((int64_t)(slot + 1) * a.n_tiles / grid) : a.n_tiles;
-    const int step = a.runs ? 1 : grid;
+    // Tile order.  In runs of consecutive tiles, the tile a workgroup
+    // prefetches is the neighbour (adjacent sequences of the same transform)
+    // of the one it stores and both touch the same HBM pages: complex64 /
+    // complex128 column passes (128-byte row segments) run ~5-10% faster.
+    // Interleaved, consecutive tiles go to different workgroups and run at
+    // the same time, which suits the transposing row pass, whose neighbours
+    // then fill whole output rows together.
+    const int K = a.runs > 0 ? a.runs : 1;  // run length of the dealt-out order
+    int tile = a.runs < 0 ? (int)((int64_t)slot * a.n_tiles / grid) : slot * K;
+    const int t_end = a.runs < 0 ? (int)((int64_t)(slot + 1) * a.n_tiles / grid) : a.n_tiles;
+    auto advance = [&](int t) { return a.runs < 0 || ((t + 1) & (K - 1)) ? t + 1 : t + 1 + (grid - 1) * K; };
     int o2 = 0, o = 0, s0 = 0, nvalid = 0;
     bool vec = false;
     if (tile < t_end) {
@@ -1122,7 +1127,7 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel_h(TileArgs a)
             pre.hr = cmul(hr, lr);
         }
         __syncthreads();
-        const int next = tile + step;
+        const int next = advance(tile);
         int no2 = 0, no = 0, ns0 = 0, nnv = 0;
         if (next < t_end) {
             decode(next, no2, no, ns0, nnv);

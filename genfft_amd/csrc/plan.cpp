@@ -275,6 +275,8 @@ struct PlanOpts {
     int pipe_mask = 3;     // pipelined kernels allowed: bit 0 column tiles, bit 1 transposing rows
     int row_variant = -1;  // registry index of the single-pass row kernel (-1: default)
     int r2c_fuse = 1;      // half-spectrum real FFTs: fused row-and-unpack pass where the shape allows
+    int col_runs = -2;     // persistent column passes' tile order (TileArgs::runs; -2: by element size)
+    int row_runs = 0;      // persistent transposing-row and single passes' tile order
 };
 
 // Picks the tile kernel for (precision, L, patterns): the registry entry
@@ -396,6 +398,8 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
     if (getenv("GFFT_ROW_M")) o.row_m = (int)env_int("GFFT_ROW_M", 0);
     if (getenv("GFFT_PIPE")) o.pipe_mask = (int)env_int("GFFT_PIPE", 3);
     if (getenv("GFFT_R2C_FUSE")) o.r2c_fuse = (int)env_int("GFFT_R2C_FUSE", 1);
+    if (getenv("GFFT_COL_RUNS")) o.col_runs = (int)env_int("GFFT_COL_RUNS", -2);
+    if (getenv("GFFT_ROW_RUNS")) o.row_runs = (int)env_int("GFFT_ROW_RUNS", 0);
     return o;
 }
 
@@ -599,7 +603,10 @@ struct Builder {
             l.grid = std::min<int64_t>(ntiles, (int64_t)std::max(1, per_cu) * std::max(1, p->num_cus));
             // contiguous runs of tiles per workgroup for the column passes of
             // 8- and 16-byte elements (measured faster); interleaved otherwise
-            a.runs = inp == PAT_COL && csize >= 8;
+            int runs = inp == PAT_COL ? (opts.col_runs == -2 ? (csize >= 8 ? -1 : 0) : opts.col_runs)
+                                      : opts.row_runs;
+            if (runs < -1 || runs == 1 || (runs > 1 && (runs & (runs - 1)))) runs = 0;  // powers of two
+            a.runs = runs;
         }
         p->launches.push_back(l);
         return GFFT_OK;

@@ -51,7 +51,7 @@ def test_column_passes_take_contiguous_tile_runs(prec):
     t = tiles(text)
     assert [field(x, "inp") for x in t] == ["1", "0"]
     for x in t:
-        want = "1" if field(x, "inp") == "1" and not store and field(x, "pipe") == "1" else "0"
+        want = "-1" if field(x, "inp") == "1" and not store and field(x, "pipe") == "1" else "0"
         assert field(x, "runs") == want, x
 
 
