@@ -15,6 +15,11 @@ The JSON line also carries
   e2e           -- the whole transform against the HBM roofline;
   copy_gbs      -- (in roofline) a plain 1 GiB device copy measured in the same run,
                    the practical ceiling of the box, with frac_of_copy beside frac;
+  pattern_copy  -- (in roofline, complex64 four-step plans) copies with no
+                   arithmetic in the HBM access pattern of each tile pass and the
+                   same pipelined loop (genfft_amd/libgfft_pattern_copy.so), in
+                   both tile orders: the ceiling a pass with that pattern can
+                   reach, and the dominant kernel's fraction of the faster one;
   cpu_baseline  -- genFFT itself (oracle/_ref, built from the reference sources)
                    timed on the host cores over a bounded sample, or the
                    oracle's C restatement when that library is absent;
@@ -154,6 +159,31 @@ def _time_ms(fn, reps, warm=2):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def pattern_copy_gbs(src, dst, n1, n2, m, mode, order):
+    """Algorithmic GB/s of a tile-pattern copy (libgfft_pattern_copy.so, mode
+    0: m-column tiles of n1 x n2 matrices read and written; mode 1: m whole
+    rows read, written transposed; order 0 interleaved tiles, 1 one run per
+    workgroup) over the float32 buffers src / dst, best of 3 x 5 runs on
+    torch's current stream."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(os.path.join(ROOT, "genfft_amd", "libgfft_pattern_copy.so"))
+    fn = lib.gfft_pattern_copy
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    fn.restype = ctypes.c_int
+    mats = src.numel() * 4 // (n1 * n2 * 8)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        rc = fn(src.data_ptr(), dst.data_ptr(), mats, n1, n2, m, mode, order, stream)
+        if rc != 0:
+            raise RuntimeError(f"gfft_pattern_copy({n1}, {n2}, {m}, mode {mode}) returned {rc}")
+
+    ms = min(_time_ms(run, 5, 2) for _ in range(3))
+    return 2.0 * mats * n1 * n2 * 8 / (ms * 1e-3) / 1e9
 
 
 def _device_ms(fn, reps, warm=2):
@@ -465,6 +495,23 @@ def main():
     dst = torch.empty_like(src)
     copy_ms = min(_time_ms(lambda: dst.copy_(src), 5, 2) for _ in range(3))
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    # the same copy in the access pattern of each complex64 four-step tile pass
+    # with 16384-element tiles (tile keys "tile L=<L> M=<M> col->col" / "row->col")
+    pattern = None
+    if prec == "c64" and kind == "c2c":
+        import re
+        pats = {}
+        for k in groups:
+            mt = re.fullmatch(r"tile L=(\d+) M=(\d+) (col->col|row->col)", k)
+            if mt and int(mt.group(1)) * int(mt.group(2)) == 16384 and n % int(mt.group(1)) == 0:
+                L_, M_ = int(mt.group(1)), int(mt.group(2))
+                shape = (L_, n // L_, M_, 0) if mt.group(3) == "col->col" else (n // L_, L_, M_, 1)
+                pats[k] = {o: pattern_copy_gbs(src, dst, *shape, i) for i, o in enumerate(("interleaved", "runs"))}
+        if pats:
+            pattern = {k: {o: round(g, 1) for o, g in v.items()} for k, v in pats.items()}
+            if dom_key in pats:
+                # against the faster of the two tile orders
+                pattern["frac_of_pattern"] = round(achieved / max(pats[dom_key].values()), 4)
     del src, dst
 
     cpu = None
@@ -504,7 +551,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_key,
                      "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
-                     "kernel_ms": round(per_launch_ms, 5), "alg_bytes_per_launch": per_launch_bytes},
+                     "kernel_ms": round(per_launch_ms, 5), "alg_bytes_per_launch": per_launch_bytes,
+                     "pattern_copy_gbs": pattern},
         "e2e": {"alg_bytes": alg_bytes, "achieved_gbs": round(alg_bytes / (ms_step * 1e-3) / 1e9, 1),
                 "hbm_frac": round(alg_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "frac_of_copy": round(alg_bytes / (ms_step * 1e-3) / 1e9 / copy_gbs, 4),

@@ -13,6 +13,9 @@
 //   mode 1  m whole rows read, written transposed into columns m*j .. m*j+m-1
 //           of the n2 x n1 result (the transposing row pass; m * n2 = 16384)
 //   mode 2  contiguous 16384-element tiles read and written
+// Tiles are visited interleaved (workgroup w takes w, w + grid, ...; order 0)
+// or in one contiguous run per workgroup (order 1), the two orders of the
+// FFT tile kernels.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -30,6 +33,7 @@ struct PcArgs {
     int32_t ntiles;
     int32_t mode;
     int32_t lg_n1, lg_n2, lg_m;  // log2 of rows, columns, tile width
+    int32_t order;               // 0 interleaved, 1 one contiguous run per workgroup
 };
 
 // element offset of access e (even) of tile t; `out` selects the write side
@@ -55,29 +59,32 @@ template <int MODE>
 __global__ void __launch_bounds__(NT) pattern_copy_kernel(PcArgs a)
 {
     f4 r[IT], q[IT];
-    int t = blockIdx.x;
-    if (t >= a.ntiles) return;
+    const int grid = gridDim.x;
+    int t = a.order ? (int)((int64_t)blockIdx.x * a.ntiles / grid) : (int)blockIdx.x;
+    const int t_end = a.order ? (int)((int64_t)(blockIdx.x + 1) * a.ntiles / grid) : a.ntiles;
+    const int step = a.order ? 1 : grid;
+    if (t >= t_end) return;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int it = 0; it < IT; it++) r[it] = a.in[pc_off<MODE>(a, t, 2 * (tid + it * NT), false) >> 1];
     while (true) {
-        int nx = t + gridDim.x;
-        if (nx < a.ntiles) {
+        int nx = t + step;
+        if (nx < t_end) {
 #pragma unroll
             for (int it = 0; it < IT; it++) q[it] = a.in[pc_off<MODE>(a, nx, 2 * (tid + it * NT), false) >> 1];
         }
 #pragma unroll
         for (int it = 0; it < IT; it++) a.out[pc_off<MODE>(a, t, 2 * (tid + it * NT), true) >> 1] = r[it];
-        if (nx >= a.ntiles) break;
+        if (nx >= t_end) break;
         t = nx;
-        nx = t + gridDim.x;
-        if (nx < a.ntiles) {
+        nx = t + step;
+        if (nx < t_end) {
 #pragma unroll
             for (int it = 0; it < IT; it++) r[it] = a.in[pc_off<MODE>(a, nx, 2 * (tid + it * NT), false) >> 1];
         }
 #pragma unroll
         for (int it = 0; it < IT; it++) a.out[pc_off<MODE>(a, t, 2 * (tid + it * NT), true) >> 1] = q[it];
-        if (nx >= a.ntiles) break;
+        if (nx >= t_end) break;
         t = nx;
     }
 }
@@ -93,15 +100,15 @@ int lg2(int64_t v)
 }  // namespace
 
 // Copies `matrices` n1 x n2 complex64 matrices from in to out (both 16-byte
-// aligned, non-overlapping) in access pattern `mode` with m-wide tiles, on
-// `stream`.  Returns 0, or -1 for a shape the pattern does not cover, or the
+// aligned, non-overlapping) in access pattern `mode` with m-wide tiles and
+// tile order `order`, on `stream`.  Returns 0, or -1 for a shape the pattern does not cover, or the
 // HIP error of the launch.
 extern "C" __attribute__((visibility("default"))) int gfft_pattern_copy(const void *in, void *out,
                                                                         long long matrices, int n1, int n2,
-                                                                        int m, int mode, void *stream)
+                                                                        int m, int mode, int order, void *stream)
 {
     const int l1 = lg2(n1), l2 = lg2(n2), lm = lg2(m);
-    if (l1 < 0 || l2 < 0 || lm < 1 || matrices <= 0 || mode < 0 || mode > 2) return -1;
+    if (l1 < 0 || l2 < 0 || lm < 1 || matrices <= 0 || mode < 0 || mode > 2 || order < 0 || order > 1) return -1;
     if (((uintptr_t)in & 15) || ((uintptr_t)out & 15)) return -1;
     if (mode == 0 && ((int64_t)n1 * m != TILE || m > n2)) return -1;
     if (mode == 1 && ((int64_t)m * n2 != TILE || m > n1)) return -1;
@@ -119,6 +126,7 @@ extern "C" __attribute__((visibility("default"))) int gfft_pattern_copy(const vo
     a.lg_n1 = l1;
     a.lg_n2 = l2;
     a.lg_m = lm;
+    a.order = order;
     const int grid = (int)(tiles < cus ? tiles : cus);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (mode == 0) hipLaunchKernelGGL(pattern_copy_kernel<0>, dim3(grid), dim3(NT), 0, s, a);

@@ -63,12 +63,12 @@ def test_pattern_copy_probe_loads_and_rejects_bad_shapes():
     lib = ctypes.CDLL(path)
     fn = lib.gfft_pattern_copy
     fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
-                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
     fn.restype = ctypes.c_int
-    for n1, n2, m, mode in ((1024, 1024, 16, 3), (1000, 1024, 16, 0), (1024, 1024, 8, 0),
-                            (1024, 1024, 32, 1), (1024, 1024, 1, 0)):
-        assert fn(None, None, 128, n1, n2, m, mode, None) == -1
-    assert fn(16, 32, 128, 1024, 1024, 16, 0, None) == -1  # pointers not 16-byte aligned
+    for n1, n2, m, mode, order in ((1024, 1024, 16, 3, 0), (1000, 1024, 16, 0, 0), (1024, 1024, 8, 0, 0),
+                                   (1024, 1024, 32, 1, 0), (1024, 1024, 1, 0, 0), (1024, 1024, 16, 0, 2)):
+        assert fn(None, None, 128, n1, n2, m, mode, order, None) == -1
+    assert fn(16, 32, 128, 1024, 1024, 16, 0, 1, None) == -1  # pointers not 16-byte aligned
 
 
 def desc(kind=C.GFFT_C2C, prec=C.GFFT_F32, n=1024, n2=0, batch=1, ins=0, outs=0, ind=0, outd=0, flags=0):
