@@ -283,7 +283,10 @@ struct C2rShape {
 // writes 64-byte row segments (complex64).  M = 32 (128-byte segments) at
 // n1 = 512 measured slower, 0.639 against 0.568 ms for 2^20 x 256: its 128 KB
 // tile leaves one workgroup per CU, and nothing overlaps its load with the
-// previous tile's store.
+// previous tile's store.  A persistent form of that tile (the next tile's
+// bins prefetched into registers, one run of tiles per workgroup) spilled
+// 52 bytes per lane and took 0.785 ms: the reload at the loop head waits for
+// every store still in flight.
 template <typename T, int N1, int M, int NT, int... RS>
 constexpr C2rShape<T> c2r_shape()
 {
