@@ -59,7 +59,9 @@ const KernelEntry kTable[] = {
     // --- persistent pipelined forms, kept where they measured faster on
     // MI355X (c64 2^20 x 128: column pass 0.565 -> 0.524 ms, transposing row
     // pass 0.462 -> 0.392 ms); the L = 512 / 2048 and M = 8 forms fall to
-    // one workgroup per CU and lose to the plain kernels ---
+    // one workgroup per CU and lose to the plain kernels (c64 2^22 x 32,
+    // L = 2048, M = 8: column pass 0.63-0.72 ms pipelined, with or without
+    // tile runs, against 0.59; transposing row pass 0.55 against 0.50) ---
     make_pipe_entry<T, 1024, 16, 1024, E_, C_, C_, 8, 8, 16>(),  // (16,16,4): ~2% slower, 12 B/lane spill
     make_pipe_entry<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
     make_pipe_entry<T, 16384, 1, 1024, S_, R_, R_, 16, 16, 16, 4>(),  // pass twiddles from global (tile: 136 KB of LDS)
