@@ -17,7 +17,7 @@ struct FilterArgs {
     int64_t batch;
     int64_t bins;         // n/2 + 1
     int32_t conj_h;       // forward kernel with conj(H) (the x-adjoint alone)
-    int32_t vec;          // all row pointers 16-byte aligned: vector loads/stores
+    int32_t pad;
 };
 
 struct ReduceArgs {

@@ -51,43 +51,6 @@ __device__ __forceinline__ void pack_pair(cx<T> yk, cx<T> yj, cx<T> w, cx<T> &zk
     zj = (yj + cconj(yk)) - mul_i(cmul(w, yj - cconj(yk)));
 }
 
-// Row <-> LDS.  complex64 rows move as 16-byte vectors (two elements per
-// lane) when every row pointer is 16-byte aligned.
-template <typename T, int L, int NT, class G>
-__device__ __forceinline__ void load_row(cx<T> *lds, const cx<T> *row, bool vec, int tid)
-{
-    if (sizeof(T) == 4 && vec) {
-        const float4 *r4 = reinterpret_cast<const float4 *>(row);
-#pragma unroll
-        for (int q = tid; q < L / 2; q += NT) {
-            const float4 v = r4[q];
-            lds[G::off(0, 2 * q)] = cx<T>{(T)v.x, (T)v.y};
-            lds[G::off(0, 2 * q + 1)] = cx<T>{(T)v.z, (T)v.w};
-        }
-    } else {
-#pragma unroll
-        for (int n = tid; n < L; n += NT) lds[G::off(0, n)] = row[n];
-    }
-}
-
-// row[n] = s * swap(lds[n]): the second FFT ran on swapped data (inverse)
-template <typename T, int L, int NT, class G>
-__device__ __forceinline__ void store_row(cx<T> *row, const cx<T> *lds, T s, bool vec, int tid)
-{
-    if (sizeof(T) == 4 && vec) {
-        float4 *r4 = reinterpret_cast<float4 *>(row);
-#pragma unroll
-        for (int q = tid; q < L / 2; q += NT) {
-            const cx<T> a = cscale(swap_ri(lds[G::off(0, 2 * q)]), s);
-            const cx<T> b = cscale(swap_ri(lds[G::off(0, 2 * q + 1)]), s);
-            r4[q] = float4{(float)a.re, (float)a.im, (float)b.re, (float)b.im};
-        }
-    } else {
-#pragma unroll
-        for (int n = tid; n < L; n += NT) row[n] = cscale(swap_ri(lds[G::off(0, n)]), s);
-    }
-}
-
 // Spectrum of a row from its packed FFT Z in LDS: bins k and L-k of pair p.
 // p = 0 yields the real bins 0 and L (as .re of fk / fj).
 template <typename T, class G>
@@ -102,6 +65,97 @@ __device__ __forceinline__ void row_bins(const cx<T> *lds, int p, int L, cx<T> w
     unpack_pair(lds[G::off(0, p)], lds[G::off(0, L - p)], w, fk, fj);
 }
 
+// The first and last Stockham passes of a row FFT against global memory.
+// One workgroup per row and per CU leaves nothing to hide a row's latency,
+// so the load is folded into the first pass (each thread reads its R1
+// inputs j + r L/R1 straight from the row: lanes run along j, coalesced) and
+// the store into the inverse's last pass, saving an LDS round trip and two
+// barriers at each end.
+//
+// Pass 1 (NS = 1, no twiddles) of M sequences, sequence s read from src(s);
+// results into LDS for the remaining passes.
+template <typename T, int L, int M, int NT, class G, int R, class Src>
+__device__ __forceinline__ void first_pass_global(cx<T> *lds, Src src, int tid)
+{
+    constexpr int LR = L / R;
+    constexpr int BF = M * LR;
+    constexpr int PER = (BF + NT - 1) / NT;
+    cx<T> v[PER][R];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int b = tid + i * NT;
+        if (BF % NT == 0 || b < BF) {
+            const cx<T> *row = src(b / LR);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[i][r] = row[b % LR + r * LR];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int b = tid + i * NT;
+        if (BF % NT == 0 || b < BF) {
+            const int s = b / LR, j = b % LR;
+            dft<R>(v[i]);
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[G::off(s, j * R + r)] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+// every pass of RS... but the last, in LDS (NS: the product of earlier radices)
+template <typename T, int L, int M, int NT, class G, int NS, int R, int... RS>
+__device__ __forceinline__ void passes_but_last(cx<T> *lds, const cx<T> *tw, int tid)
+{
+    if constexpr (sizeof...(RS) > 0) {
+        const TileArgs none{};
+        stockham_pass<T, L, M, NT, G, true, NS, R, false, false>(lds, tw, tid, none, 0, 0, Tw4Pre{});
+        constexpr int used = NS > 1 ? NS * (R - 1) : 0;
+        passes_but_last<T, L, M, NT, G, NS * R, RS...>(lds, tw + used, tid);
+    }
+}
+
+// The last pass (radix R, NS = L / R) of sequence 0 in LDS, written to `row`
+// as s * swap(v): the inverse ran on swapped data.  `tw` is the full pass
+// twiddle table of the radix sequence.
+template <typename T, int L, int NT, class G, int TWN, int R>
+__device__ __forceinline__ void last_pass_global(cx<T> *row, const cx<T> *lds, const cx<T> *tw, T scale, int tid)
+{
+    constexpr int NS = L / R;
+    constexpr int PER = (NS + NT - 1) / NT;
+    const cx<T> *twl = tw + (TWN - NS * (R - 1));
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int j = tid + i * NT;
+        if (NS % NT == 0 || j < NS) {
+            cx<T> v[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = lds[G::off(0, j + r * NS)];
+#pragma unroll
+            for (int r = 1; r < R; r++) v[r] = cmul(v[r], twl[j * (R - 1) + (r - 1)]);
+            dft<R>(v);
+#pragma unroll
+            for (int r = 0; r < R; r++) row[j + r * NS] = cscale(swap_ri(v[r]), scale);
+        }
+    }
+}
+
+template <typename T, int L, int M, int NT, class G, int R1, int... RS, class Src>
+__device__ __forceinline__ void fft_from_global(cx<T> *lds, Src src, const cx<T> *ptw, int tid)
+{
+    static_assert(sizeof...(RS) > 0, "a row FFT of at least two passes");
+    first_pass_global<T, L, M, NT, G, R1>(lds, src, tid);
+    const TileArgs none{};
+    stockham_passes<T, L, M, NT, G, true, false, R1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
+}
+
+template <typename T, int L, int NT, class G, int... RS>
+__device__ __forceinline__ void fft_to_global(cx<T> *row, cx<T> *lds, const cx<T> *ptw, T scale, int tid)
+{
+    passes_but_last<T, L, 1, NT, G, 1, RS...>(lds, ptw, tid);
+    last_pass_global<T, L, NT, G, pass_tw_count<1, RS...>(), last_radix<RS...>()>(row, lds, ptw, scale, tid);
+}
+
 template <typename T, int L, int NT, int... RS>
 __global__ void __launch_bounds__(NT) filter_fwd_kernel(FilterArgs a)
 {
@@ -114,10 +168,7 @@ __global__ void __launch_bounds__(NT) filter_fwd_kernel(FilterArgs a)
     const cx<T> *H = reinterpret_cast<const cx<T> *>(a.H);
     const cx<T> *tw = reinterpret_cast<const cx<T> *>(a.tw);
     const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
-    const TileArgs none{};
-    load_row<T, L, NT, G>(lds, x, a.vec, tid);
-    __syncthreads();
-    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
+    fft_from_global<T, L, 1, NT, G, RS...>(lds, [&](int) { return x; }, ptw, tid);
     // filter each pair of bins and pack for the inverse (stored swapped: the
     // second forward FFT then computes the unscaled inverse)
     for (int p = tid; p <= L / 2; p += NT) {
@@ -138,8 +189,7 @@ __global__ void __launch_bounds__(NT) filter_fwd_kernel(FilterArgs a)
         }
     }
     __syncthreads();
-    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
-    store_row<T, L, NT, G>(y, lds, (T)1 / (T)(2 * L), a.vec, tid);
+    fft_to_global<T, L, NT, G, RS...>(y, lds, ptw, (T)1 / (T)(2 * L), tid);
 }
 
 template <typename T, int L, int NT, int... RS>
@@ -163,11 +213,7 @@ __global__ void __launch_bounds__(NT) filter_bwd_kernel(FilterArgs a)
     const cx<T> *H = reinterpret_cast<const cx<T> *>(a.H);
     const cx<T> *tw = reinterpret_cast<const cx<T> *>(a.tw);
     const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
-    const TileArgs none{};
-    load_row<T, L, NT, G>(gl, gy, a.vec, tid);
-    load_row<T, L, NT, G>(xl, x, a.vec, tid);
-    __syncthreads();
-    stockham_passes<T, L, 2, NT, G2, true, false, 1, RS...>(lds, ptw, tid, none, 0, 0, Tw4Pre{});
+    fft_from_global<T, L, 2, NT, G2, RS...>(lds, [&](int s) { return s ? x : gy; }, ptw, tid);
     const T inv_n = (T)1 / (T)(2 * L);
     for (int p = tid; p <= L / 2; p += NT) {
         const cx<T> w = tw[p];
@@ -191,8 +237,7 @@ __global__ void __launch_bounds__(NT) filter_bwd_kernel(FilterArgs a)
         }
     }
     __syncthreads();
-    stockham_passes<T, L, 1, NT, G, true, false, 1, RS...>(gl, ptw, tid, none, 0, 0, Tw4Pre{});
-    store_row<T, L, NT, G>(gx, gl, inv_n, a.vec, tid);
+    fft_to_global<T, L, NT, G, RS...>(gx, gl, ptw, inv_n, tid);
 }
 
 // dst[y * out_pitch + k] = sum of src[b * pitch + k] over the rows b of

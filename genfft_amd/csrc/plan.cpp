@@ -2399,7 +2399,6 @@ int gfft_filter_forward(gfft_filter_plan plan, const void *x, const void *H, voi
     a.batch = plan->batch;
     a.bins = plan->n / 2 + 1;
     a.conj_h = conj_h ? 1 : 0;
-    a.vec = ((((uintptr_t)x) | ((uintptr_t)y)) & 15) == 0;
     HIP_TRY(launch_filter(plan->prec, 0, a, (hipStream_t)stream));
     return GFFT_OK;
 }
@@ -2419,7 +2418,6 @@ int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, co
     a.pass_tw = plan->pass_tw;
     a.batch = plan->batch;
     a.bins = plan->n / 2 + 1;
-    a.vec = ((((uintptr_t)x) | ((uintptr_t)gy) | ((uintptr_t)gx)) & 15) == 0;
     const hipStream_t s = (hipStream_t)stream;
     std::lock_guard<std::mutex> lk(plan->order.mu);
     HIP_TRY(plan->order.before(s));
