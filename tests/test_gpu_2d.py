@@ -74,6 +74,29 @@ def test_vert_padded_pitch(gpu, golden, prec, n, cols, ins, outs):
     assert np.all(y[:, cols:] == 7 + 5j)
 
 
+@pytest.mark.parametrize("big_in,big_out", [(True, False), (False, True)])
+def test_vert_pitch_beyond_31_bit_tile_offsets(gpu, big_in, big_out):
+    """complex64 column tiles address 16-byte vectors with 32-bit in-tile
+    offsets; a row pitch that puts a tile's last row 2^31 bytes or more from
+    its first takes the element path instead (4.3 GB buffers)."""
+    import genfft_amd as G
+    n, cols, big = 1024, 64, (1 << 19) + 2
+    assert (n - 1) * big * 8 >= 1 << 31
+    ins, outs = (big if big_in else cols), (big if big_out else cols)
+    rng = np.random.default_rng(11)
+    x = _rand((n, cols), np.complex64, rng)
+    tin = torch.empty((n - 1) * ins + cols, dtype=torch.complex64, device=gpu)
+    tin.as_strided((n, cols), (ins, 1)).copy_(torch.from_numpy(x).to(gpu))
+    out = torch.empty((n - 1) * outs + cols, dtype=torch.complex64, device=gpu)
+    f = G.FFTVert(n, torch.complex64)
+    f.transform(out, outs, tin, ins, cols, False)
+    y = out.as_strided((n, cols), (outs, 1)).cpu().numpy()
+    ref = O.fft_ld(np.ascontiguousarray(x.T).astype(np.complex128)).T
+    assert O.rel_l2_rows(y.T, ref.T).max() <= _tol("f32", n)
+    del tin, out
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_vert_matches_genfft_golden(gpu, golden, prec):
     import genfft_amd as G
