@@ -108,7 +108,9 @@ typedef struct gfft_desc {
     int64_t out_stride; /* 2D / COLS row pitch of the output, complex elements; 0 = dense */
     int64_t in_dist;    /* distance between batch members, elements of the input   */
                         /* type (complex, or real for R2C); 0 = dense               */
-    int64_t out_dist;   /* same for the output; 0 = dense                           */
+    int64_t out_dist;   /* same for the output (real for C2R); 0 = dense. A real    */
+                        /* side's distance must be even (reals move as complex      */
+                        /* pairs): GFFT_ERR_UNSUPPORTED otherwise                   */
     int32_t flags;      /* GFFT_FLAG_*                                              */
     int32_t reserved;   /* must be 0                                                */
 } gfft_desc;

@@ -55,6 +55,17 @@ def test_column_passes_take_contiguous_tile_runs(prec):
         assert field(x, "runs") == want, x
 
 
+def test_real_side_distances_must_be_even():
+    """R2C input and C2R output distances count reals, which move as complex
+    pairs: an odd distance is rejected at plan creation, an even one planned."""
+    for kind, ind, outd in ((C.GFFT_R2C, 1025, 0), (C.GFFT_C2R, 0, 1025)):
+        st, _ = explain(desc(kind=kind, n=1024, batch=3, ind=ind, outd=outd))
+        assert st == C.GFFT_ERR_UNSUPPORTED and b"even" in C.lib().gfft_last_error()
+        st, text = explain(desc(kind=kind, n=1024, batch=3, ind=ind + 1 if ind else 0,
+                                outd=outd + 1 if outd else 0))
+        assert st == C.GFFT_OK, text
+
+
 def test_pattern_copy_probe_loads_and_rejects_bad_shapes():
     """The benchmark's tile-pattern copy probe is a library of its own (not
     part of the C ABI); shapes it does not cover fail before any device call."""

@@ -1,7 +1,10 @@
 """Seeded random plan shapes against the long-double oracle: batched 1D
 complex transforms (genfft::FFT<T>::transform, include/genFFT/fft.h:80-85)
-with padded batch distances and both directions, and column transforms
-(FFTVert<T>::transform, fft.h:146-163) with random widths and row pitches.
+with padded batch distances and both directions, column transforms
+(FFTVert<T>::transform, fft.h:146-163) with random widths and row pitches,
+real-input transforms (RealFFT<T>::forward, FFTReal.h:188-221, half and
+Hermitian-filled full output) and their c2r inverse with padded distances,
+and 2D transforms (FFT2D<T>::transform, fft.h:214-218) with row pitches.
 
 The shapes reach the paths fixed sizes leave out: four-step column passes whose
 tile count is not a multiple of the persistent grid (uneven runs of tiles per
@@ -89,3 +92,99 @@ def test_random_vert_shapes(gpu, prec, n, cols, ins, outs, inv):
     ref = O.fft_ld(np.ascontiguousarray(x_np[:, :cols].T).astype(np.complex128), inv).T
     assert O.rel_l2_rows(y[:, :cols].T, ref.T).max() <= _tol(prec, n), (prec, n, cols, ins, outs, inv)
     assert np.all(y[:, cols:] == SENTINEL)
+
+
+def _real_cases():
+    """(kind, prec, n, batch, in pad, out pad, half): r2c (RealFFT::forward,
+    FFTReal.h:188-221; half or Hermitian-filled full output) and c2r."""
+    rng = np.random.default_rng(31)
+    cases = []
+    for i in range(30):
+        kind = "r2c" if i % 3 else "c2r"
+        prec = int(rng.integers(0, 2))
+        k = int(rng.integers(15, 22)) if i % 2 else int(rng.integers(1 if kind == "c2r" else 0, 15))
+        n = 1 << k
+        batch = int(rng.integers(1, max(2, min(7, (12 << 20) // n))))
+        # the real side's distance must be even (reals move as complex pairs)
+        pin, pout = int(rng.choice([0, 0, 1, 2, 5])), int(rng.choice([0, 0, 1, 4]))
+        if kind == "r2c":
+            pin += pin % 2
+        else:
+            pout += pout % 2
+        half = bool(rng.integers(0, 2)) if kind == "r2c" else True
+        cases.append((kind, prec, n, batch, pin, pout, half))
+    return cases
+
+
+def _hermitian_full(F: np.ndarray, n: int) -> np.ndarray:
+    """n bins of a real signal's spectrum from its n/2 + 1 (imaginary parts of
+    bins 0 and n/2 dropped, as a c2r does)."""
+    F = F.copy()
+    F[:, 0] = F[:, 0].real
+    F[:, n // 2] = F[:, n // 2].real
+    return np.concatenate([F, np.conj(F[:, 1:n // 2][:, ::-1])], axis=1)
+
+
+@pytest.mark.parametrize("kind,prec,n,batch,pin,pout,half", _real_cases())
+def test_random_real_shapes(gpu, kind, prec, n, batch, pin, pout, half):
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    rdt, cdt = (np.float32, np.complex64) if prec == 0 else (np.float64, np.complex128)
+    trdt, tcdt = (torch.float32, torch.complex64) if prec == 0 else (torch.float64, torch.complex128)
+    rng = np.random.default_rng(n * 13 + batch + 5 * pin + 7 * pout + half)
+    rows = sorted({0, batch // 2, batch - 1})
+    nin = n // 2 + 1
+    if kind == "r2c":
+        bins = nin if half else n
+        din, dout = n + pin, bins + pout
+        x_np = rng.uniform(-1, 1, (batch, din)).astype(rdt)
+        y = torch.full((batch * dout,), SENTINEL, dtype=tcdt, device=gpu)
+        p = G.Plan(C.GFFT_R2C, n, precision=prec, batch=batch, half=half, in_dist=din, out_dist=dout)
+        p.execute(torch.from_numpy(x_np.reshape(-1)).to(gpu), y)
+        yy = y.cpu().numpy().reshape(batch, dout)
+        ref = O.fft_ld(x_np[rows, :n].astype(np.complex128))[:, :bins]
+        assert O.rel_l2_rows(yy[rows, :bins], ref).max() <= _tol(prec, n), (kind, prec, n, batch, pin, pout, half)
+        assert np.all(yy[:, bins:] == SENTINEL)
+    else:
+        din, dout = nin + pin, n + pout
+        F = (rng.standard_normal((batch, din)) + 1j * rng.standard_normal((batch, din))).astype(cdt)
+        out = torch.full((batch * dout,), 7.0, dtype=trdt, device=gpu)
+        p = G.Plan(C.GFFT_C2R, n, precision=prec, batch=batch, in_dist=din, out_dist=dout)
+        p.execute(torch.from_numpy(F.reshape(-1)).to(gpu), out, inverse=True)
+        yy = out.cpu().numpy().reshape(batch, dout)
+        ref = O.fft_ld(_hermitian_full(F[rows, :nin].astype(np.complex128), n), inv=True).real
+        assert O.rel_l2_rows(yy[rows, :n], ref).max() <= _tol(prec, n), (kind, prec, n, batch, pin, pout)
+        assert np.all(yy[:, n:] == 7.0)
+
+
+def _2d_cases():
+    rng = np.random.default_rng(77)
+    cases = []
+    for _ in range(14):
+        prec = int(rng.integers(0, 2))
+        w, h = 1 << int(rng.integers(1, 12)), 1 << int(rng.integers(0, 12))
+        if w * h > (1 << 21):
+            h = max(1, (1 << 21) // w)
+        ins, outs = w + int(rng.choice([0, 0, 2, 7])), w + int(rng.choice([0, 0, 1, 4]))
+        inv = bool(rng.integers(0, 2))
+        cases.append((prec, w, h, ins, outs, inv))
+    return cases
+
+
+@pytest.mark.parametrize("prec,w,h,ins,outs,inv", _2d_cases())
+def test_random_2d_shapes(gpu, prec, w, h, ins, outs, inv):
+    """FFT2D<T>::transform (fft.h:214-218): rows of width w, h of them, row
+    pitches ins / outs; separable reference (rows, then columns)."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    cdt, tcdt = (np.complex64, torch.complex64) if prec == 0 else (np.complex128, torch.complex128)
+    rng = np.random.default_rng(w * 31 + h + ins + outs + inv)
+    x_np = (rng.uniform(-1, 1, (h, ins)) + 1j * rng.uniform(-1, 1, (h, ins))).astype(cdt)
+    y = torch.full((h * outs,), SENTINEL, dtype=tcdt, device=gpu)
+    p = G.Plan(C.GFFT_C2C_2D, w, precision=prec, n2=h, in_stride=ins, out_stride=outs)
+    p.execute(torch.from_numpy(x_np.reshape(-1)).to(gpu), y, inverse=inv)
+    yy = y.cpu().numpy().reshape(h, outs)
+    ref = O.fft_ld(x_np[:, :w].astype(np.complex128), inv)
+    ref = O.fft_ld(np.ascontiguousarray(ref.T), inv).T
+    assert O.rel_l2(yy[:, :w], ref) <= _tol(prec, w * h), (prec, w, h, ins, outs, inv)
+    assert np.all(yy[:, w:] == SENTINEL)
