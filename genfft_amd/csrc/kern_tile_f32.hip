@@ -62,7 +62,10 @@ const KernelEntry kTable[] = {
     // one workgroup per CU and lose to the plain kernels (c64 2^22 x 32,
     // L = 2048, M = 8: column pass 0.63-0.72 ms pipelined, with or without
     // tile runs, against 0.59; transposing row pass 0.55 against 0.50) ---
-    make_pipe_entry<T, 1024, 16, 1024, E_, C_, C_, 8, 8, 16>(),  // (16,16,4): ~2% slower, 12 B/lane spill
+    // column pass, radix 16 * 4 * 16: measured on c64 2^20 x 128 against the
+    // other orders at 1024 threads, 0.440-0.443 ms (8*8*16: 0.464, 4*16*16:
+    // 0.479, 16*16*4: 0.493; the transposing row pass does not move with order)
+    make_pipe_entry<T, 1024, 16, 1024, E_, C_, C_, 16, 4, 16>(),
     make_pipe_entry<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
     make_pipe_entry<T, 16384, 1, 1024, S_, R_, R_, 16, 16, 16, 4>(),  // pass twiddles from global (tile: 136 KB of LDS)
     // --- autotuner alternatives: radix order, tile width, threads per tile ---
