@@ -45,7 +45,7 @@ const KernelEntry kTable[] = {
     make_entry<T, 512, 16, 512, E_, C_, C_, 8, 8, 8>(),
     make_entry<T, 1024, 16, 1024, E_, C_, C_, 16, 16, 4>(),
     make_entry<T, 1024, 8, 512, E_, C_, C_, 16, 16, 4>(),
-    make_entry<T, 2048, 8, 1024, E_, C_, C_, 16, 16, 8>(),
+    make_entry<T, 2048, 8, 1024, E_, C_, C_, 16, 8, 16>(),  // c64 2^22 column pass 0.550 ms (16*16*8: 0.592, 8*16*16: 0.565)
     make_entry<T, 4096, 4, 1024, E_, C_, C_, 16, 16, 16>(),  // 32-byte rows: column transforms only, not four-step
     // --- transposing rows ---
     make_entry<T, 64, 64, 256, S_, R_, C_, 8, 8>(),
@@ -67,7 +67,10 @@ const KernelEntry kTable[] = {
     // 0.479, 16*16*4: 0.493; the transposing row pass does not move with order)
     make_pipe_entry<T, 1024, 16, 1024, E_, C_, C_, 16, 4, 16>(),
     make_pipe_entry<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
-    make_pipe_entry<T, 16384, 1, 1024, S_, R_, R_, 16, 16, 16, 4>(),  // pass twiddles from global (tile: 136 KB of LDS)
+    // single 16384-point sequences, pass twiddles from global (tile: 136 KB of
+    // LDS); radix 16*4*16*16: c64 2^14 x 8192 0.500 ms (16*16*16*4: 0.509,
+    // 16*16*4*16: 0.502, 4*16*16*16: 0.521)
+    make_pipe_entry<T, 16384, 1, 1024, S_, R_, R_, 16, 4, 16, 16>(),
     // --- autotuner alternatives: radix order, tile width, threads per tile ---
     make_variant<T, 256, 32, 512, S_, R_, R_, 16, 16>(),
     make_variant<T, 512, 8, 256, S_, R_, R_, 16, 8, 4>(),
