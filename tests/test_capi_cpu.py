@@ -66,6 +66,24 @@ def test_real_side_distances_must_be_even():
         assert st == C.GFFT_OK, text
 
 
+@pytest.mark.parametrize("prec,n,col,row", [
+    (C.GFFT_F32, 1 << 20, "16,4,16,0", "16,16,4,0"),   # measured fastest orders (MI355X)
+    (C.GFFT_F32, 1 << 22, "16,8,16,0", "16,16,8,0"),
+    (C.GFFT_F64, 1 << 22, "16,8,16,0", "16,16,8,0"),
+    (C.GFFT_F32, 1 << 14, None, "16,4,16,16"),
+])
+def test_tile_radix_orders(prec, n, col, row):
+    """The planner's kernels for the benchmark lengths carry the radix orders
+    their measurements chose (smallest radix in the middle of a column pass)."""
+    st, text = explain(desc(prec=prec, n=n, batch=2))
+    assert st == C.GFFT_OK, C.lib().gfft_last_error()
+    t = tiles(text)
+    if col is None:
+        assert len(t) == 1 and field(t[0], "rad") == row
+    else:
+        assert [field(x, "rad") for x in t] == [col, row]
+
+
 def test_pattern_copy_probe_loads_and_rejects_bad_shapes():
     """The benchmark's tile-pattern copy probe is a library of its own (not
     part of the C ABI); shapes it does not cover fail before any device call."""
