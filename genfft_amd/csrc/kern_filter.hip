@@ -9,12 +9,14 @@
 // / n with H_full the Hermitian extension of H.  gH is the gradient of a real
 // loss with respect to H in the (d/dRe + i d/dIm) convention.
 //
-// One workgroup owns one row and does everything in LDS: the n reals are read
-// as L = n/2 complex values, a forward L-point Stockham FFT gives Z, the
+// One workgroup owns one row and keeps it in LDS: the n reals are read as
+// L = n/2 complex values straight into the registers of the first pass of a
+// forward L-point Stockham FFT, which gives Z, the
 // real-FFT unpack (the adjust_DIT_impl math, generic/fft_dit_impl_generic.inl:27-61)
 // gives the half spectrum, the filter is applied pairwise (bins k and L-k
 // belong to one thread), the c2r pack is applied in the same pass, and a
-// second L-point FFT (the inverse, by the re/im swap) produces the row.  One
+// second L-point FFT (the inverse, by the re/im swap), whose last pass writes
+// to HBM, produces the row.  One
 // HBM read and one write per row; the adjoint also reads x and writes a row
 // of gH partials that a small kernel sums over the rows.
 #include "gfft_filter.h"
