@@ -607,6 +607,9 @@ struct Builder {
                                       : opts.row_runs;
             if (runs < -1 || runs == 1 || (runs > 1 && (runs & (runs - 1)))) runs = 0;  // powers of two
             a.runs = runs;
+            // the kernel steps a tile index by up to grid * run length past the last tile
+            if (ntiles + l.grid * std::max(1, runs) > INT32_MAX)
+                return fail(GFFT_ERR_UNSUPPORTED, "too many tiles (%lld)", (long long)ntiles);
         }
         p->launches.push_back(l);
         return GFFT_OK;
