@@ -10,6 +10,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from genfft_amd import _capi as C
@@ -82,6 +83,63 @@ def test_tile_radix_orders(prec, n, col, row):
         assert len(t) == 1 and field(t[0], "rad") == row
     else:
         assert [field(x, "rad") for x in t] == [col, row]
+
+
+def _random_descs(count, seed):
+    """Random plan descriptors over every kind, precision, size class, batch
+    and padded stride/distance the ABI accepts (and some it rejects)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        kind = int(rng.integers(0, 8))
+        prec = int(rng.integers(0, 4)) if kind in (C.GFFT_C2C, C.GFFT_COLS, C.GFFT_C2C_2D) else int(rng.integers(0, 2))
+        batch = int(rng.integers(1, 6))
+        pad = lambda: int(rng.choice([0, 0, 1, 2, 3, 8, 33]))
+        ins = outs = ind = outd = n2 = flags = 0
+        if kind in (C.GFFT_C2C_2D, C.GFFT_R2C_2D, C.GFFT_C2R_2D):
+            n, n2 = 1 << int(rng.integers(1, 14)), 1 << int(rng.integers(0, 14))
+            if rng.integers(0, 2):
+                ins, outs = n + pad(), n + pad()
+                if kind == C.GFFT_R2C_2D:
+                    ins += ins % 2
+                if kind == C.GFFT_C2R_2D:
+                    ins, outs = n // 2 + 1 + pad(), n + 2 * pad()
+            batch = 1
+        elif kind == C.GFFT_COLS:
+            n, n2 = 1 << int(rng.integers(0, 17)), int(rng.integers(1, 400))
+            ins, outs = n2 + pad(), n2 + pad()
+        else:
+            n = 1 << int(rng.integers(0, 24))
+            if kind == C.GFFT_DIT and rng.integers(0, 3) == 0:
+                n = 4 * int(rng.integers(1, 3000))  # genFFT's dynamic DIT: any n divisible by 4
+            flags = int(rng.integers(0, 2)) if kind in (C.GFFT_R2C, C.GFFT_DIT) else 0
+            if rng.integers(0, 2):
+                ind, outd = n + pad(), n + pad()
+                if kind == C.GFFT_C2R:
+                    ind, outd = n // 2 + 1 + pad(), n + 2 * pad()
+                elif kind == C.GFFT_R2C:
+                    ind = n + 2 * pad()
+            if n > (1 << 20):
+                batch = 1
+        out.append(desc(kind=kind, prec=prec, n=n, n2=n2, batch=batch, ins=ins, outs=outs, ind=ind, outd=outd,
+                        flags=flags))
+    return out
+
+
+def test_random_descriptors_plan_in_bounds_or_fail_cleanly():
+    """Every random descriptor either plans -- and the planner's bounds proof
+    holds for every launch -- or is rejected with a documented status; no
+    descriptor reaches an internal error."""
+    planned = 0
+    for d in _random_descs(600, 424242):
+        st, text = explain(d)
+        msg = C.lib().gfft_last_error() or b""
+        assert st in (C.GFFT_OK, C.GFFT_ERR_INVALID_ARG, C.GFFT_ERR_UNSUPPORTED), (st, msg)
+        assert b"internal" not in msg or st == C.GFFT_OK, (d.kind, d.precision, d.n, d.n2, d.batch, msg)
+        if st == C.GFFT_OK:
+            planned += 1
+            assert text.startswith("plan ")
+    assert planned > 400
 
 
 def test_pattern_copy_probe_loads_and_rejects_bad_shapes():
