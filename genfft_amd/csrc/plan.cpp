@@ -1421,7 +1421,10 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             a.swap_out = inv && l.last;
             a.vec_in = l.vin_ok && (((uintptr_t)ip & 15) == 0);
             a.vec_out = l.vout_ok && (((uintptr_t)op & 15) == 0);
-            a.xcd_remap = p->xcd_remap;
+            // workgroups that each take one contiguous run of tiles keep the
+            // hardware's block order (measured: c64 2^20 column pass 0.441 ->
+            // 0.435 ms); interleaved tiles keep neighbours on one XCD
+            a.xcd_remap = p->xcd_remap && a.runs >= 0;
             if (scale && l.in_buf == B_IN) {
                 a.scale = scale;
                 a.scale_div = scale_div;
