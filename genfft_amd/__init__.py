@@ -32,7 +32,7 @@ from ._launch import call_on as _call_on, device_index as _dev_idx, stream_handl
 __all__ = [
     "Plan", "FFT", "FFTVert", "DIT", "FFT2D", "RealFFT", "RealFFT2D", "UnevenBatch",
     "separate_2x_real_FFT", "fft", "ifft", "rfft", "irfft", "fft2", "GfftError",
-    "autotune", "save_tuning", "load_tuning",
+    "autotune", "save_tuning", "load_tuning", "clear_plan_cache",
 ]
 
 _COMPLEX = {torch.complex64: C.GFFT_F32, torch.complex128: C.GFFT_F64}
@@ -47,6 +47,30 @@ def _prec(dtype: torch.dtype) -> int:
     if dtype in _REAL:
         return _REAL[dtype]
     raise TypeError(f"unsupported dtype {dtype}: use float32/float64 or complex64/complex128")
+
+
+def _prec_complex(dtype: torch.dtype) -> int:
+    if dtype not in _COMPLEX:
+        raise TypeError(f"complex transforms take complex64 / complex128, not {dtype}")
+    return _COMPLEX[dtype]
+
+
+def _prec_real(dtype: torch.dtype) -> int:
+    if dtype not in _REAL:
+        raise TypeError(f"real transforms take float32 / float64, not {dtype}")
+    return _REAL[dtype]
+
+
+# tensor dtypes a plan's complex / real side accepts, by plan precision
+# (16-bit storage: [..., 2] float16 / bfloat16 pairs, or complex32)
+_COMPLEX_SIDE = {C.GFFT_F32: (torch.complex64,), C.GFFT_F64: (torch.complex128,),
+                 C.GFFT_F16: (torch.float16, torch.complex32), C.GFFT_BF16: (torch.bfloat16,)}
+_REAL_SIDE = {C.GFFT_F32: (torch.float32,), C.GFFT_F64: (torch.float64,)}
+
+
+def _extent(t: torch.Tensor) -> int:
+    """Bytes from t's first element to the end of its storage."""
+    return t.untyped_storage().nbytes() - t.storage_offset() * t.element_size()
 
 
 def _check_gpu(*ts: torch.Tensor) -> torch.device:
@@ -76,9 +100,35 @@ class Plan:
                 C.check(self._lib.gfft_plan_create(ctypes.byref(self._h), ctypes.byref(self.desc)))
         else:
             C.check(self._lib.gfft_plan_create(ctypes.byref(self._h), ctypes.byref(self.desc)))
+        ib, ob = ctypes.c_size_t(), ctypes.c_size_t()
+        C.check(self._lib.gfft_plan_extents(self._h, ctypes.byref(ib), ctypes.byref(ob)))
+        self.in_bytes, self.out_bytes = ib.value, ob.value
+        cplx, real = _COMPLEX_SIDE[precision], _REAL_SIDE.get(precision, ())
+        if kind in (C.GFFT_R2C, C.GFFT_R2C_2D):
+            self._in_dt, self._out_dt = real, cplx
+        elif kind in (C.GFFT_C2R, C.GFFT_C2R_2D):
+            self._in_dt, self._out_dt = cplx, real
+        elif kind == C.GFFT_DIT:  # complex data, or its interleaved scalars (the T* overload, fft.h:180-188)
+            self._in_dt = self._out_dt = cplx + real
+        else:
+            self._in_dt = self._out_dt = cplx
+
+    def _check(self, inp: torch.Tensor, out: torch.Tensor) -> torch.device:
+        """The buffers must be GPU tensors of the plan's types, each holding
+        every byte the call reads or writes (no launch ever sees a short
+        buffer)."""
+        dev = _check_gpu(inp, out)
+        for t, want, need, what in ((inp, self._in_dt, self.in_bytes, "input"),
+                                    (out, self._out_dt, self.out_bytes, "output")):
+            if t.dtype not in want:
+                raise TypeError(f"{what} dtype {t.dtype} does not match the plan ({', '.join(map(str, want))})")
+            if _extent(t) < need:
+                raise ValueError(f"{what} holds {_extent(t)} bytes but the plan {'reads' if what == 'input' else 'writes'} "
+                                 f"{need}")
+        return dev
 
     def execute(self, inp: torch.Tensor, out: torch.Tensor, inverse: bool = False, stream=None) -> None:
-        dev = _check_gpu(inp, out)
+        dev = self._check(inp, out)
         _call_on(_dev_idx(dev), stream, self._lib.gfft_execute, self._h, inp.data_ptr(), out.data_ptr(),
                  C.GFFT_INVERSE if inverse else C.GFFT_FORWARD)
 
@@ -86,7 +136,8 @@ class Plan:
                        inverse: bool = False, stream=None) -> None:
         """16-bit storage plans: out_b = scales[b] * FFT(in_b); with ``auto`` the
         library first writes power-of-two scales that keep FP16 finite."""
-        dev = _check_gpu(inp, out, scales)
+        dev = self._check(inp, out)
+        _check_gpu(inp, scales)
         if scales.dtype != torch.float32 or not scales.is_contiguous() or scales.numel() < self.desc.batch:
             raise ValueError("scales must be a contiguous float32 tensor of batch elements")
         _call_on(_dev_idx(dev), stream, self._lib.gfft_execute_scaled, self._h, inp.data_ptr(), out.data_ptr(),
@@ -96,7 +147,7 @@ class Plan:
     def execute_scrambled(self, inp: torch.Tensor, out: torch.Tensor, inverse: bool = False, stream=None) -> None:
         """The plan on input whose elements (C2C) or rows (COLS) are in
         bit-reversed order (gfft_execute_scrambled); inp may be out."""
-        dev = _check_gpu(inp, out)
+        dev = self._check(inp, out)
         _call_on(_dev_idx(dev), stream, self._lib.gfft_execute_scrambled, self._h, inp.data_ptr(), out.data_ptr(),
                  C.GFFT_INVERSE if inverse else C.GFFT_FORWARD)
 
@@ -107,7 +158,7 @@ class Plan:
     def profile(self, inp: torch.Tensor, out: torch.Tensor, inverse: bool = False, stream=None):
         """Runs once, returning the per-launch kernel durations in ms (HIP events
         recorded on the execution stream around each launch)."""
-        dev = _check_gpu(inp, out)
+        dev = self._check(inp, out)
         ms = (ctypes.c_float * 4096)()
         cnt = ctypes.c_int()
         with torch.cuda.device(dev):
@@ -191,7 +242,7 @@ def autotune(n: int, batch: int, dtype: torch.dtype = torch.complex64, reps: int
     buf = ctypes.create_string_buffer(1 << 20)
     C.check(C.lib().gfft_autotune(_prec(dtype), n, batch, reps, buf, len(buf)))
     with _cache_lock:
-        for k in [k for k in _cache if k[0] == C.GFFT_C2C and k[1] == n]:
+        for k in [k for k in (_cache or {}) if k[0] == C.GFFT_C2C and k[1] == n]:
             del _cache[k]
     return json.loads(buf.value.decode())
 
@@ -206,20 +257,43 @@ def load_tuning(path: str) -> None:
 
 # Plan cache: the analogue of the reference's per-(N, T) singleton
 # (include/genFFT/FFTLevel.h:76-92), keyed by the full descriptor and device,
-# and guarded by a lock (the reference's cache is not thread-safe).
-_cache: dict = {}
+# and guarded by a lock (the reference's cache is not thread-safe).  It holds
+# at most _CACHE_PLANS plans and _CACHE_BYTES of their device workspace,
+# dropping the least recently used first (a plan still referenced elsewhere
+# stays alive until that reference goes).
+_cache: "collections.OrderedDict" = None
 _cache_lock = threading.Lock()
+_CACHE_PLANS = 64
+_CACHE_BYTES = 4 << 30
 
 
 def _plan(kind, n, precision, device, **kw) -> Plan:
+    global _cache
+    import collections
     key = (kind, n, precision, device.index if device.index is not None else torch.cuda.current_device(),
            tuple(sorted(kw.items())))
     with _cache_lock:
+        if _cache is None:
+            _cache = collections.OrderedDict()
         p = _cache.get(key)
-        if p is None:
-            p = Plan(kind, n, precision=precision, device=device, **kw)
-            _cache[key] = p
+        if p is not None:
+            _cache.move_to_end(key)
+            return p
+        p = Plan(kind, n, precision=precision, device=device, **kw)
+        _cache[key] = p
+        held = sum(q.workspace_bytes() for q in _cache.values())
+        while len(_cache) > 1 and (len(_cache) > _CACHE_PLANS or held > _CACHE_BYTES):
+            _, old = _cache.popitem(last=False)
+            held -= old.workspace_bytes()
         return p
+
+
+def clear_plan_cache() -> None:
+    """Drops every cached plan (their device workspace is freed once no
+    caller holds them)."""
+    with _cache_lock:
+        if _cache is not None:
+            _cache.clear()
 
 
 def _batch_of(t: torch.Tensor, n: int) -> int:
@@ -240,7 +314,7 @@ class FFT:
     def __init__(self, n: int = 0, dtype: torch.dtype = torch.complex64):
         self.n = int(n)
         self.dtype = dtype
-        self._prec = _prec(dtype)
+        self._prec = _prec_complex(dtype)
 
     def size(self) -> int:
         return self.n
@@ -280,6 +354,11 @@ class FFT:
         dev = _check_gpu(out, in1, in2)
         _need_contig(out, in1, in2)
         b = _batch_of(in1, self.n)
+        rdt = _REAL_OF[self.dtype]
+        if in1.dtype != rdt or in2.dtype != rdt or out.dtype != self.dtype:
+            raise TypeError(f"transform_interleave takes {rdt} inputs and a {self.dtype} output")
+        if in2.numel() < b * self.n or out.numel() < b * self.n:
+            raise ValueError("in2 and out need as many elements as in1")
         _call_on(_dev_idx(dev), None, C.lib().gfft_interleave_real, self._prec, out.data_ptr(), in1.data_ptr(),
                  in2.data_ptr(), self.n, b)
         _plan(C.GFFT_C2C, self.n, self._prec, dev, batch=b).execute(out, out)
@@ -290,7 +369,7 @@ class RealFFT:
 
     def __init__(self, n: int = 0, dtype: torch.dtype = torch.float32):
         self.n = int(n)
-        self._prec = _prec(dtype)
+        self._prec = _prec_real(dtype)
 
     def size(self) -> int:
         return self.n
@@ -319,13 +398,14 @@ class DIT:
 
     def __init__(self, n: int = 0, dtype: torch.dtype = torch.complex64):
         self.n = int(n)
-        self._prec = _prec(dtype)
+        self._prec = _prec_complex(dtype)
 
     def size(self) -> int:
         return self.n
 
     def apply(self, out: torch.Tensor, inp: torch.Tensor, half: bool) -> None:
         dev = _check_gpu(out, inp)
+        _need_contig(out, inp)
         nin = max(1, self.n // 2)
         b = _batch_of(inp, nin)
         od = out.numel() // b
@@ -337,7 +417,7 @@ class FFTVert:
 
     def __init__(self, n: int = 0, dtype: torch.dtype = torch.complex64):
         self.n = int(n)
-        self._prec = _prec(dtype)
+        self._prec = _prec_complex(dtype)
 
     def size(self) -> int:
         return self.n
@@ -362,7 +442,7 @@ class FFT2D:
 
     def __init__(self, width: int = 0, height: int = 0, dtype: torch.dtype = torch.complex64):
         self.width, self.height = int(width), int(height)
-        self._prec = _prec(dtype)
+        self._prec = _prec_complex(dtype)
 
     def cols(self) -> int:
         return self.width
@@ -385,7 +465,8 @@ class RealFFT2D:
 
     def __init__(self, width: int = 0, height: int = 0, dtype: torch.dtype = torch.float32):
         self.width, self.height = int(width), int(height)
-        self._prec = _prec(dtype)
+        self.dtype = dtype
+        self._prec = _prec_real(dtype)
 
     def cols(self) -> int:
         return self.width
@@ -394,14 +475,35 @@ class RealFFT2D:
         return self.height
 
     def forward(self, out, out_stride, inp, in_stride) -> None:
+        """FFTReal.h:83-104: ``inp`` real, pitch ``in_stride`` scalars (any,
+        odd included); ``out`` the full complex spectrum, pitch ``out_stride``."""
         dev = _check_gpu(out, inp)
         _plan(C.GFFT_R2C_2D, self.width, self._prec, dev, n2=self.height, in_stride=int(in_stride),
               out_stride=int(out_stride)).execute(inp, out)
 
+    def forward_2x(self, out, out_stride, in1, in_stride1, in2, in_stride2) -> None:
+        """FFTReal.h:106-118: the 2D FFT of in1 + i*in2 (two real images, pitches
+        in scalars, each its own), split afterwards per element as
+        separate_2x_real_FFT does.  One launch packs every row, then the 2D
+        transform runs in place on ``out``."""
+        dev = _check_gpu(out, in1, in2)
+        cdt = _COMPLEX_OF[self.dtype]
+        if in1.dtype != self.dtype or in2.dtype != self.dtype or out.dtype != cdt:
+            raise TypeError(f"forward_2x takes {self.dtype} inputs and a {cdt} output")
+        w, h, p1, p2, po = self.width, self.height, int(in_stride1), int(in_stride2), int(out_stride)
+        for t, pitch, what in ((in1, p1, "in1"), (in2, p2, "in2"), (out, po, "out")):
+            if pitch < w:
+                raise ValueError(f"{what} pitch {pitch} is smaller than the width {w}")
+            if _extent(t) < ((h - 1) * pitch + w) * t.element_size():
+                raise ValueError(f"{what} is too small for {h} rows at pitch {pitch}")
+        _call_on(_dev_idx(dev), None, C.lib().gfft_interleave_real_2d, self._prec, out.data_ptr(), po,
+                 in1.data_ptr(), p1, in2.data_ptr(), p2, w, h)
+        _plan(C.GFFT_C2C_2D, w, self._prec, dev, n2=h, in_stride=po, out_stride=po).execute(out, out)
+
     def inverse(self, out, out_stride, inp, in_stride) -> None:
         """FFTReal.h:120-130 (a TODO in the reference): unscaled inverse of
         forward.  ``inp``: complex spectrum, columns 0..width/2 read, pitch
-        ``in_stride`` complex; ``out``: real, pitch ``out_stride`` scalars (even)."""
+        ``in_stride`` complex; ``out``: real, pitch ``out_stride`` scalars."""
         dev = _check_gpu(out, inp)
         _plan(C.GFFT_C2R_2D, self.width, self._prec, dev, n2=self.height, in_stride=int(in_stride),
               out_stride=int(out_stride)).execute(inp, out, inverse=True)
@@ -410,7 +512,12 @@ class RealFFT2D:
 def separate_2x_real_FFT(out1: torch.Tensor, out2: torch.Tensor, inp: torch.Tensor, n: int) -> None:
     """FFTReal.h:36-66 on device data (batched over rows of n)."""
     dev = _check_gpu(out1, out2, inp)
+    _prec_complex(inp.dtype)
+    if out1.dtype != inp.dtype or out2.dtype != inp.dtype:
+        raise TypeError("separate_2x_real_FFT takes three tensors of one complex dtype")
     b = _batch_of(inp, n)
+    if _extent(out1) < inp.numel() * inp.element_size() or _extent(out2) < inp.numel() * inp.element_size():
+        raise ValueError("out1 and out2 need as many elements as the input")
     _call_on(_dev_idx(dev), None, C.lib().gfft_separate_2x_real, _prec(inp.dtype), out1.data_ptr(),
              out2.data_ptr(), inp.data_ptr(), n, b)
 
@@ -431,6 +538,7 @@ def ifft(x: torch.Tensor) -> torch.Tensor:
 
 
 def rfft(x: torch.Tensor, half: bool = True) -> torch.Tensor:
+    _prec_real(x.dtype)
     x = x.contiguous()
     n = x.shape[-1]
     out = torch.empty(*x.shape[:-1], n // 2 + 1 if half else n, dtype=_COMPLEX_OF[x.dtype], device=x.device)
@@ -440,6 +548,8 @@ def rfft(x: torch.Tensor, half: bool = True) -> torch.Tensor:
 
 def irfft(X: torch.Tensor, n: int) -> torch.Tensor:
     """Unscaled c2r: n reals from n/2+1 Hermitian bins."""
+    if X.shape[-1] != n // 2 + 1:
+        raise ValueError(f"irfft of length {n} takes {n // 2 + 1} bins per row, got {X.shape[-1]}")
     X = X.contiguous()
     out = torch.empty(*X.shape[:-1], n, dtype=_REAL_OF[X.dtype], device=X.device)
     RealFFT(n, _REAL_OF[X.dtype]).inverse(out, X)

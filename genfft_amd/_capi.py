@@ -49,11 +49,13 @@ EXPORTS = (
     "gfft_execute_scaled",
     "gfft_execute_scrambled",
     "gfft_plan_workspace",
+    "gfft_plan_extents",
     "gfft_plan_describe",
     "gfft_plan_explain",
     "gfft_plan_profile",
     "gfft_separate_2x_real",
     "gfft_interleave_real",
+    "gfft_interleave_real_2d",
     "gfft_is_device_pointer",
     "gfft_error_string",
     "gfft_last_error",
@@ -131,6 +133,8 @@ def lib() -> ctypes.CDLL:
         L.gfft_execute_host.restype = i32
         L.gfft_plan_workspace.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.gfft_plan_workspace.restype = i32
+        L.gfft_plan_extents.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+        L.gfft_plan_extents.restype = i32
         L.gfft_plan_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
         L.gfft_plan_describe.restype = i32
         L.gfft_plan_explain.argtypes = [ctypes.POINTER(GfftDesc), ctypes.c_char_p, ctypes.c_size_t]
@@ -142,6 +146,8 @@ def lib() -> ctypes.CDLL:
         L.gfft_separate_2x_real.restype = i32
         L.gfft_interleave_real.argtypes = [i32, vp, vp, vp, i64, i64, vp]
         L.gfft_interleave_real.restype = i32
+        L.gfft_interleave_real_2d.argtypes = [i32, vp, i64, vp, i64, vp, i64, i64, i64, vp]
+        L.gfft_interleave_real_2d.restype = i32
         L.gfft_is_device_pointer.argtypes = [vp]
         L.gfft_is_device_pointer.restype = i32
         L.gfft_error_string.argtypes = [i32]

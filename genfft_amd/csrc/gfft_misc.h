@@ -35,11 +35,14 @@ struct SeparateArgs {
     int64_t n, batch, dist;
 };
 
+// out row r = in1 row r + i * in2 row r, n elements per row; row pitches in
+// real elements (in1, in2) and complex elements (out)
 struct InterleaveArgs {
     const void *in1;
     const void *in2;
     void *out;
     int64_t n, batch;
+    int64_t in1_pitch, in2_pitch, out_pitch;
 };
 
 struct HFillArgs {
@@ -56,6 +59,17 @@ struct CopyArgs {
     void *out;
     int64_t batch, in_dist, out_dist;
     int32_t real_in, real_out;
+};
+
+// Rows of `width` reals copied between two layouts: row r of image b starts
+// at b * in_dist + r * in_pitch in the input and b * out_dist + r * out_pitch
+// in the output (real elements).  Moves a real side whose pitch or distance
+// is odd to or from a dense copy the tile kernels can read as complex pairs.
+struct RepackArgs {
+    const void *in;
+    void *out;
+    int64_t width, rows, batch;
+    int64_t in_pitch, in_dist, out_pitch, out_dist;
 };
 
 // Per-transform scale of a 16-bit storage transform: scale[b] = 2^-k, the
@@ -92,5 +106,6 @@ hipError_t launch_separate(int prec, const SeparateArgs &a, hipStream_t s);
 hipError_t launch_interleave(int prec, const InterleaveArgs &a, hipStream_t s);
 hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s);
 hipError_t launch_copy(int prec, const CopyArgs &a, hipStream_t s);
+hipError_t launch_repack(int prec, const RepackArgs &a, hipStream_t s);
 
 }  // namespace gfft

@@ -9,6 +9,8 @@
 //   interleave  transform_interleave's packing (include/genFFT/fft.h:100-105).
 //   hfill       RealFFT2D's Hermitian column fill (include/genFFT/FFTReal.h:90-103).
 //   copy        the n = 1 transform (identity on each element).
+//   repack      rows of reals between two pitches (odd real-side layouts to
+//               and from a dense copy in scratch).
 //   amax        per-transform power-of-two scale of 16-bit storage input.
 //   bitrev      bit-reversal gather of rows (transform_no_scramble input).
 //
@@ -149,8 +151,10 @@ __global__ void __launch_bounds__(256) interleave_kernel(InterleaveArgs a)
     C *z = reinterpret_cast<C *>(a.out);
     const int64_t total = a.n * a.batch;
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x)
-        z[g] = {x[g], y[g]};
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = g / a.n, c = g - r * a.n;
+        z[r * a.out_pitch + c] = {x[r * a.in1_pitch + c], y[r * a.in2_pitch + c]};
+    }
 }
 
 // out[k][j] = conj(out[i][M - j]) for j in [vcols, M), k = (N - i) mod N.
@@ -189,6 +193,20 @@ __global__ void __launch_bounds__(256) copy_kernel(CopyArgs a)
         } else {
             out[g * a.out_dist] = in[g * a.in_dist];
         }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) repack_kernel(RepackArgs a)
+{
+    const T *in = reinterpret_cast<const T *>(a.in);
+    T *out = reinterpret_cast<T *>(a.out);
+    const int64_t per = a.width * a.rows, total = per * a.batch;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = g / per, rem = g - b * per;
+        const int64_t r = rem / a.width, c = rem - r * a.width;
+        out[b * a.out_dist + r * a.out_pitch + c] = in[b * a.in_dist + r * a.in_pitch + c];
     }
 }
 
@@ -365,6 +383,14 @@ hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s)
     const unsigned g = grid_for((a.M - a.vcols) * a.N * a.batch);
     if (prec) hipLaunchKernelGGL(hfill_kernel<double>, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(hfill_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_repack(int prec, const RepackArgs &a, hipStream_t s)
+{
+    const unsigned g = grid_for(a.width * a.rows * a.batch);
+    if (prec) hipLaunchKernelGGL(repack_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(repack_kernel<float>, dim3(g), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

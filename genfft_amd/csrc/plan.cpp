@@ -407,7 +407,8 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
 // Plan representation
 // ---------------------------------------------------------------------------
 enum BufId { B_IN = 0, B_OUT = 1, B_SCR = 2 };
-enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET, LT_R2C, LT_C2R, LT_RROWS };
+enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET, LT_R2C, LT_C2R, LT_RROWS,
+                  LT_REPACK };
 
 // bytes of one complex element in memory
 size_t csize_of(int prec, int store) { return store ? 4 : (prec ? 16 : 8); }
@@ -423,6 +424,7 @@ struct Launch {
     R2cArgs ra{};
     C2rArgs cr{};
     RealRowsArgs rr{};
+    RepackArgs rp{};
     int rr_mode = RR_R2C;
     int in_buf = B_IN, out_buf = B_OUT;
     int64_t in_off = 0, out_off = 0;  // bytes
@@ -485,12 +487,13 @@ struct gfft_plan_s {
     int num_cus = 256;
     size_t in_bytes = 0, out_bytes = 0;
     bool out_has_gaps = false;
-    // Set when a launch may write part of the output before another part of
-    // the input is read (a batch run in chunks, tiles of rows writing bins
-    // where other rows' reals lie).  Overlapping buffers are then staged
-    // (needs_stage) unless every transform is in place: in == out with equal
-    // byte distances in_step / out_step (batch members, or rows) and
-    // in_step2 / out_step2 (2D images).
+    // Set when some launch may write part of the output before all of the
+    // input is read: a launch that reads the input and writes the output
+    // (its workgroups store while others have yet to load), or an output
+    // write ahead of a later input read (a batch run in chunks).  Overlapping
+    // buffers are then staged (needs_stage) unless every transform is in
+    // place: in == out with equal byte distances in_step / out_step (batch
+    // members, or rows) and in_step2 / out_step2 (2D images).
     bool overlap_hazard = false;
     int64_t in_step = 0, out_step = 0, in_step2 = 0, out_step2 = 0;
     void *stage = nullptr;  // copy of an input that overlaps the output (allocated on first use)
@@ -660,7 +663,6 @@ struct Builder {
         const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
         int64_t chunk = std::max<int64_t>(1, chunk_bytes / (n * (int64_t)csize));
         chunk = std::min(chunk, batch);
-        if (chunk < batch) p->overlap_hazard = true;
         need_scratch(chunk * n);
         const int pm1 = opts.col_m, pm2 = opts.row_m;
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
@@ -834,7 +836,6 @@ struct Builder {
         l.out_off = out_off_el * (int64_t)csize;
         l.grid = tiles;
         p->launches.push_back(l);
-        p->overlap_hazard = true;
         return GFFT_OK;
     }
 
@@ -887,6 +888,25 @@ struct Builder {
 
 int64_t dflt(int64_t v, int64_t d) { return v ? v : d; }
 
+// Whether a launch may write the caller's output before every read of the
+// caller's input has finished: some launch reads B_IN at or after the first
+// launch that writes B_OUT (launches on one stream run in order).
+bool output_before_input_done(const std::vector<Launch> &launches)
+{
+    size_t first_write = launches.size();
+    for (size_t i = 0; i < launches.size(); i++)
+        if (launches[i].out_buf == B_OUT) {
+            first_write = i;
+            break;
+        }
+    for (size_t i = first_write; i < launches.size(); i++)
+        if (launches[i].in_buf == B_IN) return true;
+    return false;
+}
+
+int plan_via_dense_copy(gfft_plan_s *p, bool dry, const PlanOpts *forced, const gfft_desc &dense, bool input_side,
+                        const RepackArgs &rp);
+
 int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
 {
     const gfft_desc &d = p->d;
@@ -926,7 +946,11 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             p->launches.push_back(l);
             break;
         }
-        if (id % 2) return fail(GFFT_ERR_UNSUPPORTED, "R2C input distance must be even");
+        if (id % 2) {
+            gfft_desc dense = d;
+            dense.in_dist = 0;
+            return plan_via_dense_copy(p, dry, forced, dense, true, RepackArgs{nullptr, nullptr, n, batch, 1, id, 0, n, 0});
+        }
         if (b.rows_fusable(n)) {
             st = b.real_rows(RR_R2C, n, batch, B_IN, 0, id / 2, B_OUT, 0, od, half);
             break;
@@ -980,7 +1004,11 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             p->launches.push_back(l);
             break;
         }
-        if (od % 2) return fail(GFFT_ERR_UNSUPPORTED, "C2R output distance must be even");
+        if (od % 2) {
+            gfft_desc dense = d;
+            dense.out_dist = 0;
+            return plan_via_dense_copy(p, dry, forced, dense, false, RepackArgs{nullptr, nullptr, n, batch, 1, n, 0, od, 0});
+        }
         if (b.rows_fusable(n)) {
             st = b.real_rows(RR_C2R, n, batch, B_IN, 0, id, B_OUT, 0, od / 2, false);
             break;
@@ -1012,6 +1040,8 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         p->in_bytes = ((batch - 1) * id + nin) * cs;
         p->out_bytes = ((batch - 1) * od + nout) * cs;
         p->out_has_gaps = od > nout;
+        p->in_step = id * (int64_t)cs;
+        p->out_step = od * (int64_t)cs;
         p->dir_mask = 1;
         st = b.unpack(n, batch, B_IN, id, B_OUT, od, half, false);
         break;
@@ -1028,6 +1058,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         p->in_bytes = ((batch - 1) * id + (rows - 1) * ip + cols) * cs;
         p->out_bytes = ((batch - 1) * od + (rows - 1) * op + cols) * cs;
         p->out_has_gaps = op > cols || od > rows * op;
+        p->in_step = ip * (int64_t)cs;
+        p->out_step = op * (int64_t)cs;
+        p->in_step2 = id * (int64_t)cs;
+        p->out_step2 = od * (int64_t)cs;
         st = b.columns(rows, cols, batch, B_IN, ip, id, B_OUT, op, od, true, true);
         break;
     }
@@ -1044,6 +1078,10 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         p->in_bytes = ((batch - 1) * id + (H - 1) * ip + W) * cs;
         p->out_bytes = ((batch - 1) * od + (H - 1) * op + W) * cs;
         p->out_has_gaps = op > W || od > H * op;
+        p->in_step = ip * (int64_t)cs;
+        p->out_step = op * (int64_t)cs;
+        p->in_step2 = id * (int64_t)cs;
+        p->out_step2 = od * (int64_t)cs;
         if (W > max_len(p->prec, PAT_ROW, PAT_ROW, p->store))
             return fail(GFFT_ERR_UNSUPPORTED, "2D width %lld exceeds the single-pass row limit", (long long)W);
         if (W == 1) {
@@ -1073,7 +1111,11 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
             // a single real column: complex column FFT of (x, 0)
             return fail(GFFT_ERR_UNSUPPORTED, "R2C_2D needs width >= 2");
         }
-        if ((ip % 2) || (id % 2)) return fail(GFFT_ERR_UNSUPPORTED, "R2C_2D input pitch must be even");
+        if ((ip % 2) || (id % 2)) {
+            gfft_desc dense = d;
+            dense.in_stride = dense.in_dist = 0;
+            return plan_via_dense_copy(p, dry, forced, dense, true, RepackArgs{nullptr, nullptr, M, N, batch, ip, id, M, N * M});
+        }
         const int64_t half = M / 2;
         // row r2c (half spectrum, M/2+1 bins) on every row of every matrix
         if (b.rows_fusable(M)) {
@@ -1115,9 +1157,17 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         p->in_bytes = ((batch - 1) * id + (N - 1) * ip + vcols) * cs;
         p->out_bytes = ((batch - 1) * od + (N - 1) * op + M) * rs;
         p->out_has_gaps = op > M || od > N * op;
+        p->in_step = ip * (int64_t)cs;
+        p->out_step = op * (int64_t)rs;
+        p->in_step2 = id * (int64_t)cs;
+        p->out_step2 = od * (int64_t)rs;
         p->dir_mask = 2;
         if (M < 2) return fail(GFFT_ERR_UNSUPPORTED, "C2R_2D needs width >= 2");
-        if ((op % 2) || (od % 2)) return fail(GFFT_ERR_UNSUPPORTED, "C2R_2D output pitch must be even");
+        if ((op % 2) || (od % 2)) {
+            gfft_desc dense = d;
+            dense.out_stride = dense.out_dist = 0;
+            return plan_via_dense_copy(p, dry, forced, dense, false, RepackArgs{nullptr, nullptr, M, N, batch, M, N * M, op, od});
+        }
         const int64_t bsz = N * vcols;  // region B: batch x N x vcols, row pitch vcols
         const size_t c0 = p->launches.size();
         st = b.columns(N, vcols, batch, B_IN, ip, id, B_SCR, vcols, bsz, true, true);
@@ -1163,6 +1213,57 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
     if (st) return st;
     b.scratch_elems = std::max<int64_t>(b.scratch_elems, 0);
     p->scratch_bytes = (size_t)b.scratch_elems * cs;
+    p->overlap_hazard = output_before_input_done(p->launches);
+    return GFFT_OK;
+}
+
+// A real side whose pitch or distance is odd.  The kernels move reals as
+// complex pairs, so every real row must start at an even scalar offset; the
+// transform is planned instead on a dense copy of that side held in scratch
+// (`dense`: the descriptor with that side's pitches at their defaults).  For
+// an input side a repack launch fills the copy first; for an output side the
+// last launch copies the result out of it.  genFFT's RealFFT2D::forward takes
+// any scalar in_stride (include/genFFT/FFTReal.h:83, rows read at :144-165).
+int plan_via_dense_copy(gfft_plan_s *p, bool dry, const PlanOpts *forced, const gfft_desc &dense, bool input_side,
+                        const RepackArgs &rp)
+{
+    gfft_plan_s inner;
+    inner.d = dense;
+    inner.prec = p->prec;
+    inner.store = p->store;
+    inner.device = p->device;
+    inner.num_cus = p->num_cus;
+    int st = build_plan(&inner, dry, forced);
+    if (st) return st;
+    // the copy sits after everything the inner plan keeps in scratch
+    const int64_t region = (int64_t)((inner.scratch_bytes + 255) & ~(size_t)255);
+    const int moved = input_side ? B_IN : B_OUT;
+    Launch r;
+    r.type = LT_REPACK;
+    r.rp = rp;
+    r.in_buf = input_side ? B_IN : B_SCR;
+    r.in_off = input_side ? 0 : region;
+    r.out_buf = input_side ? B_SCR : B_OUT;
+    r.out_off = input_side ? region : 0;
+    p->launches.clear();
+    if (input_side) p->launches.push_back(r);
+    for (Launch l : inner.launches) {
+        if (l.in_buf == moved) {
+            l.in_buf = B_SCR;
+            l.in_off += region;
+        }
+        if (l.out_buf == moved) {
+            l.out_buf = B_SCR;
+            l.out_off += region;
+        }
+        p->launches.push_back(l);
+    }
+    if (!input_side) p->launches.push_back(r);
+    p->scratch_bytes = (size_t)region + (input_side ? inner.in_bytes : inner.out_bytes);
+    // The caller's input is read only before (input side) or the output
+    // written only after (output side) every other launch: overlapping
+    // buffers cannot be corrupted.
+    p->overlap_hazard = false;
     return GFFT_OK;
 }
 
@@ -1256,6 +1357,18 @@ int validate(const gfft_plan_s *p)
                 return st;
             break;
         }
+        case LT_REPACK: {
+            const RepackArgs &a = l.rp;
+            if (a.width < 1 || a.rows < 1 || a.batch < 1)
+                return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d repack shape", idx);
+            if ((st = inside(l.in_buf, l.in_off,
+                             ((a.batch - 1) * a.in_dist + (a.rows - 1) * a.in_pitch + a.width) * rs, "repack load")))
+                return st;
+            if ((st = inside(l.out_buf, l.out_off,
+                             ((a.batch - 1) * a.out_dist + (a.rows - 1) * a.out_pitch + a.width) * rs, "repack store")))
+                return st;
+            break;
+        }
         case LT_COPY: {
             const CopyArgs &a = l.ca;
             const int64_t isz = a.real_in ? rs : cs, osz = a.real_out ? rs : cs;
@@ -1335,6 +1448,13 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
         case LT_COPY:
             snprintf(line, sizeof(line), "copy batch=%lld in=%s+%lld out=%s+%lld\n", (long long)l.ca.batch,
                      bn[l.in_buf], (long long)l.in_off, bn[l.out_buf], (long long)l.out_off);
+            break;
+        case LT_REPACK:
+            snprintf(line, sizeof(line), "repack width=%lld rows=%lld batch=%lld in=%s+%lld out=%s+%lld "
+                     "in_pitch=%lld in_dist=%lld out_pitch=%lld out_dist=%lld\n", (long long)l.rp.width,
+                     (long long)l.rp.rows, (long long)l.rp.batch, bn[l.in_buf], (long long)l.in_off, bn[l.out_buf],
+                     (long long)l.out_off, (long long)l.rp.in_pitch, (long long)l.rp.in_dist,
+                     (long long)l.rp.out_pitch, (long long)l.rp.out_dist);
             break;
         case LT_MIX:
             snprintf(line, sizeof(line), "mix tasks=%d grid=%lld\n", l.mix.n_tasks, (long long)l.grid);
@@ -1480,6 +1600,13 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             a.in = ip;
             a.out = op;
             HIP_TRY(launch_copy(p->prec, a, s));
+            break;
+        }
+        case LT_REPACK: {
+            RepackArgs a = l.rp;
+            a.in = ip;
+            a.out = op;
+            HIP_TRY(launch_repack(p->prec, a, s));
             break;
         }
         case LT_MIX_RESET:
@@ -2169,6 +2296,14 @@ int gfft_plan_workspace(gfft_plan plan, size_t *bytes)
     return GFFT_OK;
 }
 
+int gfft_plan_extents(gfft_plan plan, size_t *in_bytes, size_t *out_bytes)
+{
+    if (!plan || !in_bytes || !out_bytes) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    *in_bytes = plan->in_bytes;
+    *out_bytes = plan->out_bytes;
+    return GFFT_OK;
+}
+
 int gfft_plan_describe(gfft_plan plan, char *buf, size_t len)
 {
     if (!plan || !buf || !len) return fail(GFFT_ERR_INVALID_ARG, "null argument");
@@ -2188,7 +2323,19 @@ int gfft_interleave_real(int precision, void *out, const void *in1, const void *
                          void *stream)
 {
     if (!out || !in1 || !in2 || n < 1 || batch < 1) return fail(GFFT_ERR_INVALID_ARG, "bad argument");
-    InterleaveArgs a{in1, in2, out, n, batch};
+    InterleaveArgs a{in1, in2, out, n, batch, n, n, n};
+    HIP_TRY(launch_interleave(precision, a, (hipStream_t)stream));
+    return GFFT_OK;
+}
+
+int gfft_interleave_real_2d(int precision, void *out, int64_t out_stride, const void *in1, int64_t in_stride1,
+                            const void *in2, int64_t in_stride2, int64_t width, int64_t height, void *stream)
+{
+    if (!out || !in1 || !in2 || width < 1 || height < 1) return fail(GFFT_ERR_INVALID_ARG, "bad argument");
+    if (precision != GFFT_F32 && precision != GFFT_F64) return fail(GFFT_ERR_INVALID_ARG, "bad precision");
+    if (out_stride < width || in_stride1 < width || in_stride2 < width)
+        return fail(GFFT_ERR_INVALID_ARG, "row pitch smaller than the width");
+    InterleaveArgs a{in1, in2, out, width, height, in_stride1, in_stride2, out_stride};
     HIP_TRY(launch_interleave(precision, a, (hipStream_t)stream));
     return GFFT_OK;
 }

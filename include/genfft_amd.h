@@ -73,7 +73,7 @@ extern "C" {
 #define GFFT_C2R_2D 7   /* 2D inverse of GFFT_R2C_2D (RealFFT2D::inverse): n = width */
                         /* (reals per row), n2 = height; reads columns 0..n/2 of   */
                         /* the complex input (pitch in complex), writes reals      */
-                        /* (pitch in scalars, even); unscaled                      */
+                        /* (pitch in scalars); unscaled                            */
 
 /* precisions */
 #define GFFT_F32 0      /* complex64  (interleaved float)  */
@@ -108,9 +108,10 @@ typedef struct gfft_desc {
     int64_t out_stride; /* 2D / COLS row pitch of the output, complex elements; 0 = dense */
     int64_t in_dist;    /* distance between batch members, elements of the input   */
                         /* type (complex, or real for R2C); 0 = dense               */
-    int64_t out_dist;   /* same for the output (real for C2R); 0 = dense. A real    */
-                        /* side's distance must be even (reals move as complex      */
-                        /* pairs): GFFT_ERR_UNSUPPORTED otherwise                   */
+    int64_t out_dist;   /* same for the output (real for C2R); 0 = dense.  A real   */
+                        /* side may have any pitch and distance: when one is odd   */
+                        /* the kernels (which move reals as complex pairs) run on  */
+                        /* a dense copy in the plan's workspace, one extra pass    */
     int32_t flags;      /* GFFT_FLAG_*                                              */
     int32_t reserved;   /* must be 0                                                */
 } gfft_desc;
@@ -126,14 +127,14 @@ GFFT_API int gfft_plan_create(gfft_plan *plan, const gfft_desc *desc);
 GFFT_API int gfft_plan_destroy(gfft_plan plan);
 
 /* Executes on device-resident buffers, asynchronously on `stream`
- * (a hipStream_t; NULL = the null stream).  in == out is allowed for every
- * kind except GFFT_COLS / GFFT_C2C_2D / GFFT_R2C_2D / GFFT_C2R_2D with different
- * pitches (C2R_2D also needs out != in: its output is real).  Where a plan's
- * launches write part of the output before reading all of the input (a 1D
- * batch beyond the 1 GiB workspace run in chunks, real transforms of rows
- * that fit one tile) and the buffers overlap other than as in == out with
- * equal input and output distances in bytes, the input is first copied to a
- * plan-owned buffer (allocated on first use) and the plan runs from there.
+ * (a hipStream_t; NULL = the null stream).  Input and output may overlap in
+ * any way.  Where a plan's launches may write part of the output before all
+ * of the input is read (every plan whose tile pass reads the input and
+ * writes the output, a 1D batch beyond the 1 GiB workspace run in chunks)
+ * and the buffers overlap other than exactly in place (in == out with equal
+ * input and output pitches and distances in bytes), the input is first
+ * copied to a plan-owned buffer (allocated on first use) and the plan runs
+ * from there.
  * direction: GFFT_FORWARD or GFFT_INVERSE (R2C / R2C_2D / DIT are forward
  * only, C2R inverse only).  Calls may share a plan from any threads and
  * streams, as with genFFT's shared plans (FFTLevel.h:76-92): a call on a
@@ -180,6 +181,12 @@ GFFT_API int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int di
 /* Device workspace the plan holds, in bytes. */
 GFFT_API int gfft_plan_workspace(gfft_plan plan, size_t *bytes);
 
+/* Bytes a call reads from `in` and writes to `out` (from each pointer to the
+ * last byte touched, pitch gaps included): what a binding checks its buffers
+ * against before it calls gfft_execute.  No reference counterpart (genFFT's
+ * callers size their own arrays). */
+GFFT_API int gfft_plan_extents(gfft_plan plan, size_t *in_bytes, size_t *out_bytes);
+
 /* Human-readable list of the plan's kernel launches (for logs and tests). */
 GFFT_API int gfft_plan_describe(gfft_plan plan, char *buf, size_t len);
 
@@ -198,6 +205,16 @@ GFFT_API int gfft_separate_2x_real(int precision, void *out1, void *out2, const 
  * each, batch rows at distance n. */
 GFFT_API int gfft_interleave_real(int precision, void *out, const void *in1, const void *in2, int64_t n,
                          int64_t batch, void *stream);
+
+/* RealFFT2D<T>::forward_2x's packing step (include/genFFT/FFTReal.h:106-118,
+ * rows scrambled at :167-180) in one launch: row r of out (pitch out_stride,
+ * complex elements) = row r of in1 + i * row r of in2 (pitches in_stride1 /
+ * in_stride2, real elements), width elements per row, height rows.  Each
+ * input keeps its own pitch (genFFT offsets in2's second half by in_stride1,
+ * FFTReal.h:178, which agrees only when the two pitches are equal). */
+GFFT_API int gfft_interleave_real_2d(int precision, void *out, int64_t out_stride, const void *in1,
+                                     int64_t in_stride1, const void *in2, int64_t in_stride2, int64_t width,
+                                     int64_t height, void *stream);
 
 /* ---- uneven batches (no reference counterpart: genFFT runs one transform
  * per call) ------------------------------------------------------------------

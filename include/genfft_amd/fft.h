@@ -25,6 +25,13 @@
 //     (README.txt:28-31); transform_many<inv>(out, in, batch) runs a batch;
 //   * RealFFT<T>::inverse(out, in) is the unscaled half-spectrum -> real
 //     transform the reference lacks (FFTReal.h:127-130, README.txt:51-52).
+//
+// The backend seam keeps the reference's spelling: FFT, FFTVert, DIT and
+// RealFFT take the factory template parameters of fft.h:56, :117, :173 and
+// FFTReal.h:188, and the factory names of every reference backend
+// (backend::, impl_generic::, impl_native::, impl_x86_dispatch::, all
+// FFTBackend.h:40-70) name the gfx950 engine's factories, so code that picks
+// a CPU tier compiles unchanged and runs on the GPU.
 #ifndef GENFFT_AMD_FFT_H
 #define GENFFT_AMD_FFT_H
 
@@ -142,11 +149,112 @@ inline int64_t bitrev(int64_t i, int64_t n)
 /// Stream used for device-pointer calls (a hipStream_t; nullptr = null stream).
 inline void set_stream(void *hip_stream) { detail::stream_ref() = hip_stream; }
 
-/// A 1D FFT for densely packed data (include/genFFT/fft.h:54-113).
+// ---- the backend seam (include/genFFT/fft.h:41-52, FFTBackend.h:40-70) ----
+namespace impl {
+/// The reference's backend objects hold a CPU tier's butterflies
+/// (FFTLevel.h:43-117, FFTDIT.h:40-50).  Every class here runs on the gfx950
+/// engine, so a factory's result records only the size it accepted; the
+/// plans themselves come from the engine's plan cache when a class runs.
 template <class T>
+struct FFTBase {
+    int n = 0;
+};
+template <class T>
+struct FFTVertBase {
+    int n = 0;
+};
+template <class T>
+struct FFTDITBase {
+    int n = 0;
+};
+}  // namespace impl
+
+template <typename T>
+using FFTImplPtr = std::shared_ptr<impl::FFTBase<T>>;
+template <typename T>
+using FFTVertImplPtr = std::shared_ptr<impl::FFTVertBase<T>>;
+template <typename T>
+using FFTDITImplPtr = std::shared_ptr<impl::FFTDITBase<T>>;
+template <typename T>
+using FFTImplFactory = FFTImplPtr<T>(int n, T);
+template <typename T>
+using FFTVertImplFactory = FFTVertImplPtr<T>(int n, T);
+template <typename T>
+using FFTDITImplFactory = FFTDITImplPtr<T>(int n, T);
+
+namespace detail {
+inline bool pow2_size(int n) { return n >= 1 && n <= (1 << 23) && (n & (n - 1)) == 0; }
+
+template <class B>
+std::shared_ptr<B> make_impl(int n, bool ok, const char *what)
+{
+    if (!ok) throw error(GFFT_ERR_UNSUPPORTED, std::string("genfft_amd: unsupported ") + what + " size " + std::to_string(n));
+    auto p = std::make_shared<B>();
+    p->n = n;
+    return p;
+}
+}  // namespace detail
+
+/// The gfx950 engine's factories, one overload per scalar type as the
+/// reference's (generic/fft_impl_generic.h:245-283, fft_dit_impl_generic.inl:117-153).
+namespace impl_gfx950 {
+inline FFTImplPtr<float> GetImpl(int n, float)
+{
+    return detail::make_impl<impl::FFTBase<float>>(n, detail::pow2_size(n), "FFT");
+}
+inline FFTImplPtr<double> GetImpl(int n, double)
+{
+    return detail::make_impl<impl::FFTBase<double>>(n, detail::pow2_size(n), "FFT");
+}
+inline FFTVertImplPtr<float> GetVertImpl(int n, float)
+{
+    return detail::make_impl<impl::FFTVertBase<float>>(n, detail::pow2_size(n), "FFTVert");
+}
+inline FFTVertImplPtr<double> GetVertImpl(int n, double)
+{
+    return detail::make_impl<impl::FFTVertBase<double>>(n, detail::pow2_size(n), "FFTVert");
+}
+// DIT: powers of two and, as genFFT's dynamic FFTDIT<-1>, any n divisible by 4
+inline FFTDITImplPtr<float> GetDITImpl(int n, float)
+{
+    return detail::make_impl<impl::FFTDITBase<float>>(n, detail::pow2_size(n) || (n > 0 && n % 4 == 0), "DIT");
+}
+inline FFTDITImplPtr<double> GetDITImpl(int n, double)
+{
+    return detail::make_impl<impl::FFTDITBase<double>>(n, detail::pow2_size(n) || (n > 0 && n % 4 == 0), "DIT");
+}
+}  // namespace impl_gfx950
+
+namespace backend {
+using impl_gfx950::GetDITImpl;
+using impl_gfx950::GetImpl;
+using impl_gfx950::GetVertImpl;
+}  // namespace backend
+namespace impl_generic {
+using impl_gfx950::GetDITImpl;
+using impl_gfx950::GetImpl;
+using impl_gfx950::GetVertImpl;
+}  // namespace impl_generic
+namespace impl_native {
+using impl_gfx950::GetDITImpl;
+using impl_gfx950::GetImpl;
+using impl_gfx950::GetVertImpl;
+}  // namespace impl_native
+namespace impl_x86_dispatch {
+using impl_gfx950::GetDITImpl;
+using impl_gfx950::GetImpl;
+using impl_gfx950::GetVertImpl;
+}  // namespace impl_x86_dispatch
+
+/// A 1D FFT for densely packed data (include/genFFT/fft.h:54-113).
+template <class T, FFTImplFactory<T> *factory = backend::GetImpl>
 struct FFT {
     FFT() = default;
-    explicit FFT(int n) : n(n), plan(detail::get_plan(GFFT_C2C, detail::precision<T>(), n, 0, 1, 0, 0, 0, 0, 0)) {}
+    explicit FFT(int n)
+        : n(n), impl(factory(n, T())),
+          plan(detail::get_plan(GFFT_C2C, detail::precision<T>(), n, 0, 1, 0, 0, 0, 0, 0))
+    {
+    }
 
     /// fft.h:80-85: natural-order transform; the inverse is unscaled.
     template <bool inv>
@@ -201,19 +309,20 @@ struct FFT {
     }
 
     int size() const noexcept { return n; }
-    explicit operator bool() const noexcept { return (bool)plan; }
+    explicit operator bool() const noexcept { return (bool)impl; }
 
 private:
     int n = 0;
+    FFTImplPtr<T> impl;
     detail::PlanPtr plan;
 };
 
 /// Column-wise 1D FFT for multiple columns (include/genFFT/fft.h:115-171).
 /// Strides are in complex elements.
-template <class T>
+template <class T, FFTVertImplFactory<T> *factory = backend::GetVertImpl>
 struct FFTVert {
     FFTVert() = default;
-    explicit FFTVert(int n) : n(n) { valid = true; }
+    explicit FFTVert(int n) : n(n), impl(factory(n, T())) {}
 
     template <bool inv>
     void transform(std::complex<T> *out, stride_t out_stride, const std::complex<T> *in, stride_t in_stride,
@@ -247,19 +356,19 @@ struct FFTVert {
     }
 
     int size() const noexcept { return n; }
-    explicit operator bool() const noexcept { return valid; }
+    explicit operator bool() const noexcept { return (bool)impl; }
 
 private:
     int n = 0;
-    bool valid = false;
+    FFTVertImplPtr<T> impl;
 };
 
 /// Real-FFT unpack step (include/genFFT/fft.h:173-196).
-template <class T>
+template <class T, FFTDITImplFactory<T> *factory = backend::GetDITImpl>
 class DIT {
 public:
     DIT() = default;
-    explicit DIT(int n) : n(n) { valid = true; }
+    explicit DIT(int n) : n(n), impl(factory(n, T())) {}
 
     void apply(T *out, const T *in, bool half)
     {
@@ -269,11 +378,11 @@ public:
     void apply(std::complex<T> *out, const std::complex<T> *in, bool half) { apply((T *)out, (const T *)in, half); }
 
     int size() const noexcept { return n; }
-    explicit operator bool() const noexcept { return valid; }
+    explicit operator bool() const noexcept { return (bool)impl; }
 
 private:
     int n = 0;
-    bool valid = false;
+    FFTDITImplPtr<T> impl;
 };
 
 /// 2D FFT (include/genFFT/fft.h:198-245); strides in complex elements.
@@ -341,10 +450,9 @@ public:
     {
         const bool dev = detail::on_device(in1, out);
         if (dev) {
-            for (int r = 0; r < h; r++)
-                detail::check(gfft_interleave_real(detail::precision<T>(), out + (size_t)r * out_stride,
-                                                   in1 + (size_t)r * in_stride1, in2 + (size_t)r * in_stride2, w, 1,
-                                                   detail::stream_ref()));
+            // one launch packs every row, then the 2D transform runs in place
+            detail::check(gfft_interleave_real_2d(detail::precision<T>(), out, out_stride, in1, in_stride1, in2,
+                                                  in_stride2, w, h, detail::stream_ref()));
         } else {
             for (int r = 0; r < h; r++)
                 for (int c = 0; c < w; c++)
@@ -357,7 +465,7 @@ public:
     /// FFTReal.h:120-130 (the reference's body is assert(!"TODO")): the
     /// unscaled inverse of forward -- reads columns 0..width/2 of the complex
     /// spectrum `in` (pitch in complex elements), writes width x height reals
-    /// (pitch in scalars, even); inverse(forward(x)) = width * height * x.
+    /// (pitch in scalars); inverse(forward(x)) = width * height * x.
     void inverse(T *out, int out_stride, const std::complex<T> *in, int in_stride)
     {
         auto p = detail::get_plan(GFFT_C2R_2D, detail::precision<T>(), w, h, 1, in_stride, out_stride, 0, 0, 0);
@@ -371,11 +479,20 @@ private:
     int w = 0, h = 0;
 };
 
-/// Real-input 1D FFT (include/genFFT/FFTReal.h:186-221).
-template <class T>
+/// Real-input 1D FFT (include/genFFT/FFTReal.h:186-221).  The factories are
+/// consulted as the reference does (the n/2-point transform and the n-point
+/// unpack); the transform itself is one engine plan of the whole real FFT.
+template <class T, FFTImplFactory<T> *factory = backend::GetImpl,
+          FFTDITImplFactory<T> *dit_factory = backend::GetDITImpl>
 struct RealFFT {
     RealFFT() = default;
-    RealFFT(int n) : n(n) {}
+    RealFFT(int n) : n(n)
+    {
+        if (n > 1) {
+            impl = factory(n / 2, T());
+            dit = dit_factory(n, T());
+        }
+    }
 
     /// half: write n/2+1 bins; otherwise all n (upper half by conjugate symmetry).
     void forward(std::complex<T> *out, const T *in, bool half)
@@ -394,6 +511,8 @@ struct RealFFT {
     int size() const noexcept { return n; }
 
 private:
+    FFTImplPtr<T> impl;
+    FFTDITImplPtr<T> dit;
     int n = 0;
 };
 

@@ -154,6 +154,12 @@ REF_API void ref_real_forward_batch_mt(void *out, int64_t out_dist, const void *
     {                                                                                              \
         genfft::RealFFT2D<T> f(width, height);                                                     \
         f.forward(reinterpret_cast<std::complex<T> *>(out), os, in, is);                          \
+    }                                                                                              \
+    REF_API void ref_real2d_forward_2x_##SFX(T *out, int os, const T *in1, int is1, const T *in2,  \
+                                             int is2, int width, int height)                       \
+    {                                                                                              \
+        genfft::RealFFT2D<T> f(width, height);                                                     \
+        f.forward_2x(reinterpret_cast<std::complex<T> *>(out), os, in1, is1, in2, is2);           \
     }
 
 REF_TYPED(float, f32)

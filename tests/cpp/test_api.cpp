@@ -7,9 +7,10 @@
 //   test/test_dispatch.cpp:84-126  FFT_vert_test_pow2 table (every column checked)
 //   test/test_real_fft.cpp:42-67   FFT_real_test half/full + sentinel, 2 .. 2^22
 //   test/test_dit.cpp:45-67        FFT_DIT_test_pow2 in/out of place, 2 .. 2^20
-// plus FFT2D / RealFFT2D / transform_real / transform_interleave /
-// separate_2x_real_FFT, which the reference leaves untested, and the
-// device-pointer path.
+// plus FFT2D / RealFFT2D (forward with odd input pitches, forward_2x) /
+// transform_real / transform_interleave / separate_2x_real_FFT, which the
+// reference leaves untested, the factory template parameters spelled with the
+// reference's backend names, and the device-pointer path.
 //
 // Inputs: DummyData exactly as test/test_util.h:36-57 (std::mt19937_64 with
 // the default seed, std::uniform_real_distribution<T>(-1, 1)).  Expected
@@ -262,6 +263,94 @@ void TestRealFFT2D(int w, int h)
     g_pass++;
 }
 
+// RealFFT2D::forward with an odd scalar input pitch (FFTReal.h:83 takes any
+// in_stride) and forward_2x (FFTReal.h:106-118) on host and device data, each
+// input at its own pitch, against the 2D transform of in1 + i*in2.
+template <typename T>
+void TestRealFFT2D_Pitches(int w, int h, int p1, int p2, int po)
+{
+    genfft::RealFFT2D<T> fft(w, h);
+    std::vector<T> a((size_t)h * p1), b((size_t)h * p2);
+    DummyData(a);
+    std::mt19937_64 rng(11);
+    std::uniform_real_distribution<T> dist(-1, 1);
+    for (auto &v : b) v = dist(rng);
+    std::vector<std::complex<T>> ca((size_t)h * w), cz((size_t)h * w);
+    for (int r = 0; r < h; r++)
+        for (int c = 0; c < w; c++) {
+            ca[(size_t)r * w + c] = a[(size_t)r * p1 + c];
+            cz[(size_t)r * w + c] = std::complex<T>(a[(size_t)r * p1 + c], b[(size_t)r * p2 + c]);
+        }
+    genfft::FFT2D<T> cfft(w, h);
+    std::vector<std::complex<T>> ra((size_t)h * w), rz((size_t)h * w);
+    cfft.template transform<false>(ra.data(), w, ca.data(), w);
+    cfft.template transform<false>(rz.data(), w, cz.data(), w);
+    const std::complex<T> fill(7, 5);
+    std::vector<std::complex<T>> out((size_t)h * po, fill), got((size_t)h * w);
+    auto dense = [&](const std::vector<std::complex<T>> &v) {
+        for (int r = 0; r < h; r++)
+            for (int c = 0; c < w; c++) got[(size_t)r * w + c] = v[(size_t)r * po + c];
+        bool gaps = true;
+        for (int r = 0; r < h; r++)
+            for (int c = w; c < po; c++) gaps = gaps && v[(size_t)r * po + c] == fill;
+        return gaps;
+    };
+    fft.forward(out.data(), po, a.data(), p1);
+    EXPECT(dense(out) && rel_l2(got.data(), ra.data(), w * h) <= 2 * RelTol<T>(w * h),
+           "RealFFT2D::forward %dx%d in_stride %d", w, h, p1);
+    std::fill(out.begin(), out.end(), fill);
+    fft.forward_2x(out.data(), po, a.data(), p1, b.data(), p2);
+    EXPECT(dense(out) && rel_l2(got.data(), rz.data(), w * h) <= 2 * RelTol<T>(w * h),
+           "RealFFT2D::forward_2x host %dx%d pitches %d %d", w, h, p1, p2);
+    // device data: one packing launch, then the 2D transform in place
+    void *da = nullptr, *db = nullptr, *dz = nullptr;
+    EXPECT(hipMalloc(&da, a.size() * sizeof(T)) == hipSuccess && hipMalloc(&db, b.size() * sizeof(T)) == hipSuccess &&
+               hipMalloc(&dz, out.size() * sizeof(std::complex<T>)) == hipSuccess, "hipMalloc");
+    std::fill(out.begin(), out.end(), fill);
+    EXPECT(hipMemcpy(da, a.data(), a.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(db, b.data(), b.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(dz, out.data(), out.size() * sizeof(std::complex<T>), hipMemcpyHostToDevice) == hipSuccess,
+           "H2D");
+    fft.forward_2x((std::complex<T> *)dz, po, (const T *)da, p1, (const T *)db, p2);
+    EXPECT(hipMemcpy(out.data(), dz, out.size() * sizeof(std::complex<T>), hipMemcpyDeviceToHost) == hipSuccess, "D2H");
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dz);
+    EXPECT(dense(out) && rel_l2(got.data(), rz.data(), w * h) <= 2 * RelTol<T>(w * h),
+           "RealFFT2D::forward_2x device %dx%d pitches %d %d", w, h, p1, p2);
+    g_pass++;
+}
+
+// The reference's factory template parameters (fft.h:56, :117, :173,
+// FFTReal.h:188): code that names a CPU backend compiles and runs on the GPU.
+template <typename T>
+void TestFactorySpellings(int n)
+{
+    std::vector<std::complex<T>> in(n), out(n), ref(n);
+    DummyData(in, false);
+    genfft::FFT<T, genfft::impl_generic::GetImpl> f(n);
+    EXPECT((bool)f && f.size() == n, "FFT<T, impl_generic::GetImpl> plan");
+    f.template transform<false>(out.data(), in.data());
+    genfft::FFT<T> g(n);
+    g.template transform<false>(ref.data(), in.data());
+    bool same = true;
+    for (int i = 0; i < n; i++) same = same && out[i] == ref[i];
+    EXPECT(same, "FFT<T, impl_generic::GetImpl> and FFT<T> agree bit for bit, n=%d", n);
+    genfft::RealFFT<T, genfft::impl_native::GetImpl, genfft::impl_native::GetDITImpl> rf(n);
+    std::vector<T> x(n);
+    DummyData(x);
+    std::vector<std::complex<T>> h(n / 2 + 1);
+    rf.forward(h.data(), x.data(), true);
+    std::vector<std::complex<T>> cx(n);
+    for (int i = 0; i < n; i++) cx[i] = x[i];
+    auto rx = oracle(cx.data(), n, false);
+    EXPECT(rel_l2(h.data(), rx.data(), n / 2 + 1) <= RelTol<T>(n), "RealFFT<T, impl_native...> n=%d", n);
+    genfft::FFTVert<T, genfft::impl_x86_dispatch::GetVertImpl> v(n);
+    genfft::DIT<T, genfft::backend::GetDITImpl> d(n);
+    EXPECT((bool)v && (bool)d, "FFTVert / DIT with named factories");
+    g_pass++;
+}
+
 template <typename T>
 void TestRealHelpers(int n)
 {
@@ -380,6 +469,11 @@ int main(int argc, char **argv)
     TestFFT2D<float>(256, 4096, 256, 260);
     TestRealFFT2D<float>(32, 16);
     TestRealFFT2D<double>(64, 128);
+    TestRealFFT2D_Pitches<float>(32, 16, 33, 35, 34);
+    TestRealFFT2D_Pitches<double>(64, 8, 64, 67, 64);
+    TestRealFFT2D_Pitches<float>(256, 64, 257, 256, 260);
+    TestFactorySpellings<float>(1024);
+    TestFactorySpellings<double>(1 << 16);
     for (int n : {2, 16, 1024, 1 << 15}) {
         TestRealHelpers<float>(n);
         TestRealHelpers<double>(n);

@@ -11,7 +11,10 @@ on the hot path.  The fixture holds data only: seeded inputs and the outputs of
   - FFT<T>::transform_real / transform_interleave (fft.h:90-105)
   - FFTVert<T>::transform         (fft.h:146-150), incl. odd column counts
   - FFT2D<T>::transform           (fft.h:214-218), padded pitches
-  - RealFFT2D<T>::forward         (FFTReal.h:83-104)
+  - RealFFT2D<T>::forward         (FFTReal.h:83-104), incl. odd scalar input pitches
+  - RealFFT2D<T>::forward_2x      (FFTReal.h:106-118), equal input pitches only:
+                                  genFFT offsets in2's second half by in_stride1
+                                  (FFTReal.h:178), right only when the pitches agree
 Usage:  python tests/golden/make_golden.py
 """
 import ctypes
@@ -111,6 +114,26 @@ def main():
             out[key + "_in"] = x
             y = np.full(h * outs, 7 + 5j, dt)
             getattr(r, "ref_real2d_forward_" + sfx)(P(y), outs, P(x), ins, w, h)
+            out[key + "_fwd"] = y
+        # own generator, so the inputs of every earlier entry stay as they were
+        rng2 = np.random.default_rng(4242 if sfx == "f32" else 4243)
+        # RealFFT2D::forward with odd scalar input pitches (FFTReal.h:83)
+        for (w, h, ins, outs) in [(32, 16, 33, 35), (8, 8, 9, 8), (64, 4, 65, 64), (16, 1, 17, 16)]:
+            x = rng2.uniform(-1, 1, h * ins).astype(rdt)
+            key = f"real2d_{sfx}_w{w}_h{h}_i{ins}_o{outs}"
+            out[key + "_in"] = x
+            y = np.full(h * outs, 7 + 5j, dt)
+            getattr(r, "ref_real2d_forward_" + sfx)(P(y), outs, P(x), ins, w, h)
+            out[key + "_fwd"] = y
+        # RealFFT2D::forward_2x, both inputs at one pitch (odd and even)
+        for (w, h, ins, outs) in [(32, 16, 33, 35), (16, 8, 16, 16), (8, 32, 9, 8), (64, 2, 64, 66)]:
+            x1 = rng2.uniform(-1, 1, h * ins).astype(rdt)
+            x2 = rng2.uniform(-1, 1, h * ins).astype(rdt)
+            key = f"real2d2x_{sfx}_w{w}_h{h}_i{ins}_o{outs}"
+            out[key + "_in1"] = x1
+            out[key + "_in2"] = x2
+            y = np.full(h * outs, 7 + 5j, dt)
+            getattr(r, "ref_real2d_forward_2x_" + sfx)(P(y), outs, P(x1), ins, P(x2), ins, w, h)
             out[key + "_fwd"] = y
     path = os.path.join(HERE, "genfft_golden.npz")
     np.savez_compressed(path, **out)
