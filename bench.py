@@ -85,7 +85,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the fused-filter, uneven-batch and precision legs")
-    ap.add_argument("--leg", choices=["fused_filter", "uneven_batch", "precision"],
+    ap.add_argument("--leg", choices=["fused_filter", "uneven_batch", "precision", "reference_cases"],
                     help="run only this leg and print its JSON object (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (scale reference for PMC byte counters)")
@@ -95,8 +95,16 @@ def parse():
 def tile_launches(text):
     out = []
     for ln in text.splitlines():
+        if ln.startswith("dual"):
+            # two tile passes in one launch: "dual ... | <role A fields> | <role B fields>"
+            parts = [p.strip() for p in ln.split("|")]
+            f = dict(kv.split("=", 1) for kv in parts[0].split()[1:] if "=" in kv)
+            f["kind"] = "dual"
+            f["roles"] = [dict(kv.split("=", 1) for kv in p.split() if "=" in kv) for p in parts[1:]]
+            out.append(f)
+            continue
         if not ln.startswith(("tile", "r2c_rows_unpack", "c2r_pack_cols", "real_rows", "unpack", "pack", "hfill",
-                              "copy")):
+                              "copy", "repack")):
             continue
         f = dict(kv.split("=", 1) for kv in ln.split()[1:] if "=" in kv)
         f["kind"] = ln.split()[0]
@@ -104,14 +112,41 @@ def tile_launches(text):
     return out
 
 
+def _tile_key(f):
+    return f"tile L={f['L']} M={f['M']} {'row' if f['inp'] == '0' else 'col'}->{'row' if f['outp'] == '0' else 'col'}"
+
+
+def _tile_elems(f):
+    return int(f["n_inner"]) * int(f["tiles"]) // int(f["tiles_inner"]) * int(f["L"])
+
+
+def host_cpus():
+    """(usable, host): the CPUs this process may run on -- its affinity mask,
+    capped by a cgroup CPU quota when one is set -- and the host's count."""
+    host = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = host
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            usable = min(usable, max(1, math.ceil(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, usable), host
+
+
 def cpu_baseline(x_cpu, n, batch, prec, gpu_out_cpu):
-    """genFFT (or the oracle's restatement) on the host cores, bounded sample."""
+    """genFFT (or the oracle's restatement) on every CPU this process may use,
+    bounded sample."""
     import ctypes
 
     import numpy as np
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libgenfft_ref.so")
     orc_so = os.path.join(ROOT, "oracle", "liboracle.so")
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores, host = host_cpus()
     sample = min(batch, 128)
     xs = np.ascontiguousarray(x_cpu[:sample])
     ys = np.empty_like(xs)
@@ -139,12 +174,54 @@ def cpu_baseline(x_cpu, n, batch, prec, gpu_out_cpu):
         "value": round(flops / best / 1e9, 3),
         "unit": "GFLOP/s",
         "cores": cores,
+        "host_cores": host,
         "kind": kind,
         "sample": f"{sample} forward transforms of the same N={n} batch (first {sample} rows), "
-                  f"best of 3, batch strided over {cores} threads sharing one plan",
+                  f"best of 3, batch strided over {cores} threads sharing one plan (one thread per CPU "
+                  f"this process may use: affinity mask and cgroup quota; the host has {host})",
         "seconds": round(best, 4),
         "gpu_vs_cpu_rel_l2_max": agree,
     }
+
+
+def accuracy(kind, x, y, n, n2, prec, inv=False):
+    """Relative L2 error of spot transforms (first, middle, last of the batch;
+    the whole image for 2D) against numpy's float64 FFT of the same input --
+    an independent reference (the long-double oracle runs in the tests)."""
+    import numpy as np
+    import torch
+
+    def host(t):
+        t = t.detach()
+        if t.dtype in (torch.float16, torch.bfloat16):  # [.., 2] pairs
+            t = torch.view_as_complex(t.double().contiguous())
+        return t.cpu().numpy().astype(np.complex128 if t.is_complex() else np.float64)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    if kind in ("c2c_2d", "r2c_2d", "c2r_2d"):
+        xi, yi = host(x).reshape(n2, -1), host(y).reshape(n2, -1)
+        if kind == "c2c_2d":
+            ref = np.fft.fft2(xi)
+        elif kind == "r2c_2d":
+            ref = np.fft.fft2(xi)
+        else:
+            ref = np.fft.irfft2(xi, s=(n2, n)) * (n * n2)
+        return {"rel_l2_max": rel(yi, ref), "spot": "whole image", "reference": "numpy.fft float64"}
+    b = x.shape[0]
+    rows = sorted({0, b // 2, b - 1})
+    errs = []
+    for r in rows:
+        xr, yr = host(x[r]), host(y[r])
+        if kind == "r2c":
+            ref = np.fft.rfft(xr.real)
+        elif kind == "c2r":
+            ref = np.fft.irfft(xr, n) * n
+        else:
+            ref = np.fft.ifft(xr) * n if inv else np.fft.fft(xr)
+        errs.append(rel(yr.reshape(ref.shape), ref))
+    return {"rel_l2_max": max(errs), "spot": f"transforms {rows}", "reference": "numpy.fft float64"}
 
 
 def _time_ms(fn, reps, warm=2):
@@ -277,11 +354,108 @@ def bench_uneven(dev):
     b = G.UnevenBatch(sizes, offs, offs, inv, torch.complex64)
     ms = {m: _time_ms(lambda: b.execute(x, y, mode=m), r, 1) for m, r in
           (("graph", 20), ("eager", 20), ("per_job", 3))}
+    # 500 of the 20,000 jobs against numpy's float64 FFT (y holds the last run's output)
+    b.execute(x, y, mode="graph")
+    torch.cuda.synchronize()
+    xs, ys = x.cpu().numpy().astype(np.complex128), y.cpu().numpy().astype(np.complex128)
+    worst = 0.0
+    for j in np.random.default_rng(1).choice(count, 500, replace=False):
+        o, m = int(offs[j]), int(sizes[j])
+        ref = np.fft.ifft(xs[o:o + m]) * m if inv[j] else np.fft.fft(xs[o:o + m])
+        worst = max(worst, float(np.linalg.norm(ys[o:o + m] - ref) / np.linalg.norm(ref)))
     return {"config": f"{count} complex64 transforms, sizes 2^6..2^16 (seed 0), 50/50 forward/inverse, "
                       f"{total} elements", "graph_ms": round(ms["graph"], 4), "eager_ms": round(ms["eager"], 4),
             "per_job_ms": round(ms["per_job"], 4),
             "speedup_vs_per_job": round(ms["per_job"] / ms["graph"], 2),
+            "rel_l2_max_500_jobs": worst, "accuracy_reference": "numpy.fft float64, 500 random jobs, bound "
+                                                                 "2e-7*log2(n) <= 3.2e-6",
             "alg_gbs_graph": round(total * 16 / (ms["graph"] * 1e-3) / 1e9, 1)}
+
+
+def _vert_args():
+    """(nfft, ncols) of test/fft_bench.cpp:155-165 (ncols += 1 up to 16, then
+    ncols + 3; nfft * ncols < 2^22), every third ncols of each nfft kept."""
+    out = []
+    nfft = 1
+    while nfft <= (1 << 20):
+        cols, ncols = [], 1
+        while ncols * nfft < (1 << 22):
+            cols.append(ncols)
+            ncols += 1 if ncols <= 16 else ncols + 3
+        out += [(nfft, c) for c in cols[::3]]
+        nfft *= 4
+    return out
+
+
+def bench_reference_cases(dev):
+    """The reference's own benchmark cases (test/fft_bench.cpp:41-176), one
+    transform per call as genFFT runs them:
+      FFT_1D      forward + unscaled inverse of one complex64 transform, n = 2 .. 2^21
+      RealFFT_1D  one real float32 transform to n/2 + 1 bins, n = 2 .. 2^21
+      FFT_Vert    forward + inverse column FFT, nfft x ncols (subsampled sweep)
+    each on host memory (the drop-in path genFFT callers take: staged through
+    the device, synchronous) and on device memory (HIP events around the
+    calls).  err: the reference's own check, mean |inverse(forward(x))/n - x|."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    lib = C.lib()
+    rng = np.random.default_rng(0)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+
+    def host_us(fn, reps):
+        fn()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            best = min(best, time.perf_counter() - t0)
+        return best * 1e6
+
+    def case(kind, n, n2, x_np, out_np, inv_plan=None):
+        """(host us, device us, err) for one forward (+ inverse) pair."""
+        plan = G.Plan(kind, n, n2=n2, half=kind == C.GFFT_R2C)
+        back_np = np.empty_like(x_np) if inv_plan else None
+
+        def host():
+            C.check(lib.gfft_execute_host(plan._h, P(x_np), P(out_np), 0))
+            if inv_plan:
+                C.check(lib.gfft_execute_host(plan._h, P(out_np), P(back_np), 1))
+        reps = 20 if x_np.nbytes < (1 << 22) else 5
+        h = host_us(host, reps)
+        err = float(np.mean(np.abs(back_np / n - x_np))) if inv_plan else None
+        xd = torch.from_numpy(x_np).to(dev)
+        yd = torch.empty(out_np.shape, dtype=torch.from_numpy(out_np).dtype, device=dev)
+        zd = torch.empty_like(xd) if inv_plan else None
+
+        def device():
+            plan.execute(xd, yd)
+            if inv_plan:
+                plan.execute(yd, zd, inverse=True)
+        d = _time_ms(device, 50, 3) * 1e3
+        del plan
+        return round(h, 2), round(d, 2), err
+
+    out = {"timing": "host_us: best of 5-20 synchronous calls on host buffers (H2D + transform + D2H, the "
+                     "genFFT drop-in's memory model); device_us: mean of 50 calls on device buffers, HIP events",
+           "FFT_1D": [], "RealFFT_1D": [], "FFT_Vert": []}
+    for k in range(1, 22):
+        n = 1 << k
+        x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex64)
+        h, d, err = case(C.GFFT_C2C, n, 0, x, np.empty_like(x), inv_plan=True)
+        out["FFT_1D"].append({"n": n, "host_us": h, "device_us": d, "err": err})
+        r = rng.uniform(-1, 1, n).astype(np.float32)
+        h, d, _ = case(C.GFFT_R2C, n, 0, r, np.empty(n // 2 + 1, np.complex64))
+        out["RealFFT_1D"].append({"n": n, "host_us": h, "device_us": d})
+    for nfft, ncols in _vert_args():
+        x = (rng.uniform(-1, 1, nfft * ncols) + 1j * rng.uniform(-1, 1, nfft * ncols)).astype(np.complex64)
+        h, d, err = case(C.GFFT_COLS, nfft, ncols, x, np.empty_like(x), inv_plan=True)
+        out["FFT_Vert"].append({"nfft": nfft, "ncols": ncols, "host_us": h, "device_us": d, "err": err})
+    return out
 
 
 def bench_precision(dev):
@@ -351,7 +525,8 @@ def main():
     from genfft_amd import _capi as C
 
     if a.leg:
-        leg = {"fused_filter": bench_filter, "uneven_batch": bench_uneven, "precision": bench_precision}[a.leg]
+        leg = {"fused_filter": bench_filter, "uneven_batch": bench_uneven, "precision": bench_precision,
+               "reference_cases": bench_reference_cases}[a.leg]
         print(json.dumps({a.leg: leg(torch.device("cuda", 0))}), flush=True)
         return
     kind, n2 = "c2c", 0
@@ -454,9 +629,13 @@ def main():
     groups = {}
     for l, t in zip(launches, acc):
         if l["kind"] == "tile":
-            key = f"tile L={l['L']} M={l['M']} {'row' if l['inp'] == '0' else 'col'}->" \
-                  f"{'row' if l['outp'] == '0' else 'col'}"
-            elems = int(l["n_inner"]) * int(l["tiles"]) // int(l["tiles_inner"]) * int(l["L"])
+            key = _tile_key(l)
+            elems = _tile_elems(l)
+        elif l["kind"] == "dual":
+            # column pass of one chunk beside the row pass of the previous one:
+            # each role reads and writes its own elements
+            key = "dual " + " + ".join(_tile_key(r) for r in l["roles"])
+            elems = sum(_tile_elems(r) for r in l["roles"])
         elif l["kind"] == "real_rows":
             # rows of n = 2L reals <-> bins in one pass (unpack / pack fused)
             key = f"real rows {l['mode']} L={l['L']} M={l['M']}"
@@ -514,6 +693,9 @@ def main():
                 pattern["frac_of_pattern"] = round(achieved / max(pats[dom_key].values()), 4)
     del src, dst
 
+    acc_line = accuracy(kind, x, y, n, n2, prec, inv) if not half else None
+    if half:  # 16-bit storage: against the float64 FFT of the stored (dequantized) input
+        acc_line = accuracy("c2c", x, y, n, n2, prec)
     cpu = None
     if not a.no_cpu and not half and kind == "c2c":
         cpu = cpu_baseline(x.cpu().numpy(), n, batch, prec, y.cpu().numpy())
@@ -524,6 +706,7 @@ def main():
         extras["fused_filter"] = bench_filter(dev)
         extras["uneven_batch"] = bench_uneven(dev)
         extras["precision"] = bench_precision(dev)
+        extras["reference_cases"] = bench_reference_cases(dev)
 
     line = {
         "metric": "batched c2c fp32 FFT GFLOP/s & %HBM peak at N=2^20; fused fwd+adjoint speedup",
@@ -560,6 +743,7 @@ def main():
                             for k, v in groups.items()},
                 "wall_s": round(wall, 3)},
         "cpu_baseline": cpu,
+        "accuracy": acc_line,
     }
     line.update(extras)
     print(json.dumps(line), flush=True)

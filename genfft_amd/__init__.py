@@ -103,11 +103,13 @@ class Plan:
         ib, ob = ctypes.c_size_t(), ctypes.c_size_t()
         C.check(self._lib.gfft_plan_extents(self._h, ctypes.byref(ib), ctypes.byref(ob)))
         self.in_bytes, self.out_bytes = ib.value, ob.value
+        # a real side also takes a complex tensor of the same precision (its
+        # interleaved scalars: real transforms in place over the spectrum)
         cplx, real = _COMPLEX_SIDE[precision], _REAL_SIDE.get(precision, ())
         if kind in (C.GFFT_R2C, C.GFFT_R2C_2D):
-            self._in_dt, self._out_dt = real, cplx
+            self._in_dt, self._out_dt = real + cplx, cplx
         elif kind in (C.GFFT_C2R, C.GFFT_C2R_2D):
-            self._in_dt, self._out_dt = cplx, real
+            self._in_dt, self._out_dt = cplx, real + cplx
         elif kind == C.GFFT_DIT:  # complex data, or its interleaved scalars (the T* overload, fft.h:180-188)
             self._in_dt = self._out_dt = cplx + real
         else:

@@ -871,24 +871,35 @@ __device__ __forceinline__ void put_tile(cx<T> *lds, const cx<T> (&reg)[M * L / 
     }
 }
 
+// LDS footprint of a pipelined tile shape: the tile, then the pass twiddles
+// when both fit a CU's 160 KB.
 template <typename T, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
-__global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
+struct PipeLds {
+    using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
+    static constexpr int TWN = pass_tw_count<1, RS...>();
+    static constexpr bool TWL = pass_tw_in_lds<T, G::ELEMS, TWN>();
+    static constexpr int TILE = G::ELEMS;
+    static constexpr int TW = TWL && TWN > 0 ? TWN : 1;
+};
+
+// The persistent loop over a launch's tiles, as workgroup `block` of `grid`
+// workgroups working on them (a kernel may hand other workgroups other work).
+// `lds` holds the tile and `twl` the pass twiddles (PipeLds).
+template <typename T, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
+__device__ __forceinline__ void pipe_tiles(const TileArgs &a, int block, int grid, cx<T> *lds, cx<T> *twl)
 {
     using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
     constexpr int TWN = pass_tw_count<1, RS...>();
     constexpr int E = M * L / NT;
     constexpr int LAST_R = last_radix<RS...>();
     constexpr bool TWL = pass_tw_in_lds<T, G::ELEMS, TWN>();
-    __shared__ cx<T> lds[G::ELEMS];
-    __shared__ cx<T> twl[TWL && TWN > 0 ? TWN : 1];
     const int tid = threadIdx.x;
     const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
     if constexpr (TWL) {
         for (int i = tid; i < TWN; i += NT) twl[i] = ptw[i];
         ptw = twl;
     }
-    const int grid = gridDim.x;
-    const int slot = a.xcd_remap ? xcd_tile(blockIdx.x, grid) : (int)blockIdx.x;
+    const int slot = a.xcd_remap ? xcd_tile(block, grid) : block;
     auto decode = [&](int t, int &o2, int &o, int &s0, int &nv) {
         const int t2 = t / a.tiles_inner;
         const int ti = t - t2 * a.tiles_inner;
@@ -968,6 +979,48 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
         s0 = ns0;
         nvalid = nnv;
     }
+}
+
+template <typename T, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
+__global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
+{
+    using S = PipeLds<T, L, M, NT, LAYOUT, INP, OUTP, RS...>;
+    __shared__ cx<T> lds[S::TILE];
+    __shared__ cx<T> twl[S::TW];
+    pipe_tiles<T, L, M, NT, LAYOUT, INP, OUTP, RS...>(a, (int)blockIdx.x, (int)gridDim.x, lds, twl);
+}
+
+// A pipelined tile shape as one type (for kernels that combine two).
+template <typename T_, int L, int M, int NT_, int LAYOUT, int INP, int OUTP, int... RS>
+struct PipeShape {
+    using T = T_;
+    static constexpr int NT = NT_;
+    using S = PipeLds<T_, L, M, NT_, LAYOUT, INP, OUTP, RS...>;
+    static constexpr int ELEMS = S::TILE + S::TW;
+    static __device__ __forceinline__ void run(const TileArgs &a, int block, int grid, cx<T_> *smem)
+    {
+        pipe_tiles<T_, L, M, NT_, LAYOUT, INP, OUTP, RS...>(a, block, grid, smem, smem + S::TILE);
+    }
+};
+
+// Two tile passes of different shapes in one launch: workgroups [0, ga) run
+// pass A's tiles, the rest run pass B's.  The four-step transform runs its
+// batch in chunks through a ring of two scratch slots, and each launch
+// pairs the column pass of chunk k with the transposing row pass of chunk
+// k - 1: the two passes' HBM streams (reads of the input, writes of the
+// output, the scratch ring in between) overlap instead of following each
+// other.  The choice of role is uniform per workgroup, so the barriers
+// inside each role's loop see every thread of the workgroup.
+template <class A, class B>
+__global__ void __launch_bounds__(A::NT) dual_pipe_kernel(TileArgs a, TileArgs b, int ga)
+{
+    static_assert(A::NT == B::NT, "both roles run with the kernel's thread count");
+    static_assert(sizeof(typename A::T) == sizeof(typename B::T), "one element type");
+    using T = typename A::T;
+    constexpr int E = A::ELEMS > B::ELEMS ? A::ELEMS : B::ELEMS;
+    __shared__ cx<T> smem[E];
+    if ((int)blockIdx.x < ga) A::run(a, (int)blockIdx.x, ga, smem);
+    else B::run(b, (int)blockIdx.x - ga, (int)gridDim.x - ga, smem);
 }
 
 // ---------------------------------------------------------------------------

@@ -115,9 +115,39 @@ constexpr KernelEntry make_pipe_entry_h()
     return e;
 }
 
+// A dual_pipe_kernel pairing two pipelined tile passes: role A (the four-step
+// column pass of one chunk) and role B (the transposing row pass of the
+// chunk before), identified by the single-pass kernels they replace.
+struct DualEntry {
+    int prec;
+    int NT;
+    int lds_bytes;
+    void (*a_fn)(TileArgs);  // the tile_fft_pipe_kernel whose tiles role A runs
+    void (*b_fn)(TileArgs);  // the same for role B
+    void (*fn)(TileArgs, TileArgs, int);
+};
+
+template <class A, class B>
+constexpr DualEntry make_dual(void (*a_fn)(TileArgs), void (*b_fn)(TileArgs))
+{
+    static_assert(A::NT == B::NT, "one thread count");
+    constexpr int e = A::ELEMS > B::ELEMS ? A::ELEMS : B::ELEMS;
+    static_assert(e * sizeof(cx<typename A::T>) <= 160 * 1024, "both roles fit gfx950 LDS");
+    DualEntry d{};
+    d.prec = sizeof(typename A::T) == 8 ? 1 : 0;
+    d.NT = A::NT;
+    d.lds_bytes = (int)(e * sizeof(cx<typename A::T>));
+    d.a_fn = a_fn;
+    d.b_fn = b_fn;
+    d.fn = &dual_pipe_kernel<A, B>;
+    return d;
+}
+
 // Defined in kern_tile_f32.hip / kern_tile_f64.hip / kern_tile_h.hip.
 const KernelEntry *tile_kernels_f32(int *count);
 const KernelEntry *tile_kernels_f64(int *count);
+const DualEntry *dual_kernels_f32(int *count);
+const DualEntry *dual_kernels_f64(int *count);
 const KernelEntry *tile_kernels_f16(int *count);
 const KernelEntry *tile_kernels_bf16(int *count);
 

@@ -88,12 +88,26 @@ const KernelEntry kTable[] = {
     make_variant<T, 8192, 1, 512, S_, R_, R_, 16, 16, 16, 2>(),
     make_variant<T, 16384, 1, 1024, S_, R_, R_, 16, 16, 8, 8>(),
 };
+// The 2^20 four-step's two pipelined passes paired in one launch (column pass
+// of chunk k, transposing row pass of chunk k - 1).
+using ColPass = PipeShape<T, 1024, 16, 1024, E_, C_, C_, 16, 4, 16>;
+using RowPass = PipeShape<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>;
+const DualEntry kDual[] = {
+    make_dual<ColPass, RowPass>(&tile_fft_pipe_kernel<T, 1024, 16, 1024, E_, C_, C_, 16, 4, 16>,
+                                &tile_fft_pipe_kernel<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>),
+};
 }  // namespace
 
 const KernelEntry *tile_kernels_f32(int *count)
 {
     *count = (int)(sizeof(kTable) / sizeof(kTable[0]));
     return kTable;
+}
+
+const DualEntry *dual_kernels_f32(int *count)
+{
+    *count = (int)(sizeof(kDual) / sizeof(kDual[0]));
+    return kDual;
 }
 
 }  // namespace gfft

@@ -68,12 +68,26 @@ const KernelEntry kTable[] = {
     make_variant<T, 4096, 1, 256, S_, R_, R_, 16, 16, 16>(),
     make_variant<T, 8192, 1, 512, S_, R_, R_, 16, 16, 8, 4>(),
 };
+// The 2^20 four-step's two pipelined passes paired in one launch (column pass
+// of chunk k, transposing row pass of chunk k - 1).
+using ColPass = PipeShape<T, 1024, 8, 512, E_, C_, C_, 16, 16, 4>;
+using RowPass = PipeShape<T, 1024, 8, 512, S_, R_, C_, 16, 16, 4>;
+const DualEntry kDual[] = {
+    make_dual<ColPass, RowPass>(&tile_fft_pipe_kernel<T, 1024, 8, 512, E_, C_, C_, 16, 16, 4>,
+                                &tile_fft_pipe_kernel<T, 1024, 8, 512, S_, R_, C_, 16, 16, 4>),
+};
 }  // namespace
 
 const KernelEntry *tile_kernels_f64(int *count)
 {
     *count = (int)(sizeof(kTable) / sizeof(kTable[0]));
     return kTable;
+}
+
+const DualEntry *dual_kernels_f64(int *count)
+{
+    *count = (int)(sizeof(kDual) / sizeof(kDual[0]));
+    return kDual;
 }
 
 }  // namespace gfft

@@ -277,6 +277,7 @@ struct PlanOpts {
     int r2c_fuse = 1;      // half-spectrum real FFTs: fused row-and-unpack pass where the shape allows
     int col_runs = -2;     // persistent column passes' tile order (TileArgs::runs; -2: by element size)
     int row_runs = 0;      // persistent transposing-row and single passes' tile order
+    int dual = 0;          // four-step batches in chunks, column pass of chunk k beside row pass of k - 1
 };
 
 // Picks the tile kernel for (precision, L, patterns): the registry entry
@@ -310,6 +311,17 @@ const KernelEntry *find_tile(int prec, int L, int inp, int outp, int prefer_m = 
         if (better(e)) best = &e;
     }
     return best;
+}
+
+// The dual kernel pairing two pipelined tile kernels (role A, role B), if any.
+const DualEntry *find_dual(int prec, const KernelEntry *a, const KernelEntry *b)
+{
+    if (!a || !b || !a->persistent || !b->persistent || a->store || b->store) return nullptr;
+    int cnt = 0;
+    const DualEntry *tab = prec ? dual_kernels_f64(&cnt) : dual_kernels_f32(&cnt);
+    for (int i = 0; i < cnt; i++)
+        if (tab[i].a_fn == a->fn && tab[i].b_fn == b->fn) return &tab[i];
+    return nullptr;
 }
 
 int max_len(int prec, int inp, int outp, int store = 0)
@@ -400,6 +412,7 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
     if (getenv("GFFT_R2C_FUSE")) o.r2c_fuse = (int)env_int("GFFT_R2C_FUSE", 1);
     if (getenv("GFFT_COL_RUNS")) o.col_runs = (int)env_int("GFFT_COL_RUNS", -2);
     if (getenv("GFFT_ROW_RUNS")) o.row_runs = (int)env_int("GFFT_ROW_RUNS", 0);
+    if (getenv("GFFT_DUAL")) o.dual = (int)env_int("GFFT_DUAL", 0);
     return o;
 }
 
@@ -408,7 +421,7 @@ PlanOpts resolve_opts(int prec, int64_t n, const PlanOpts *forced)
 // ---------------------------------------------------------------------------
 enum BufId { B_IN = 0, B_OUT = 1, B_SCR = 2 };
 enum LaunchType { LT_TILE, LT_UNPACK, LT_PACK, LT_HFILL, LT_COPY, LT_MIX, LT_MIX_RESET, LT_R2C, LT_C2R, LT_RROWS,
-                  LT_REPACK };
+                  LT_REPACK, LT_DUAL };
 
 // bytes of one complex element in memory
 size_t csize_of(int prec, int store) { return store ? 4 : (prec ? 16 : 8); }
@@ -431,6 +444,18 @@ struct Launch {
     bool vin_ok = false, vout_ok = false;
     bool first = false, last = false;  // ends of a complex chain (inverse swaps)
     int64_t grid = 0;
+    // LT_DUAL: a second tile pass in the same launch (role B; role A is the
+    // tile pass described by the fields above), run by the workgroups from
+    // `split` on
+    const DualEntry *dual = nullptr;
+    const KernelEntry *kb = nullptr;
+    TileArgs tb{};
+    int in_buf_b = B_IN, out_buf_b = B_OUT;
+    int64_t in_off_b = 0, out_off_b = 0;
+    bool vin_ok_b = false, vout_ok_b = false, first_b = false, last_b = false;
+    int64_t split = 0;
+    bool reads(int buf) const { return in_buf == buf || (type == LT_DUAL && in_buf_b == buf); }
+    bool writes(int buf) const { return out_buf == buf || (type == LT_DUAL && out_buf_b == buf); }
 };
 
 }  // namespace
@@ -528,6 +553,10 @@ struct Builder {
     int store = 0;              // 16-bit storage format (tile_kernels_f16 / _bf16), default shapes only
 
     void need_scratch(int64_t elems) { scratch_elems = std::max(scratch_elems, elems); }
+
+    // Largest batch chunk a four-step plan runs through its workspace: 1 GiB
+    // (GFFT_CHUNK_BYTES overrides).
+    int64_t chunk_bytes() const { return env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30); }
 
     // one tile pass
     int tile(int L, int inp, int outp, int prefer_m, int in_buf, int64_t in_off_el, int out_buf,
@@ -656,12 +685,16 @@ struct Builder {
         const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
         if (N1 > max_len(prec, PAT_COL, PAT_COL, store) || N2 > max_len(prec, PAT_ROW, PAT_COL, store))
             return fail(GFFT_ERR_UNSUPPORTED, "length %lld exceeds the two-pass limit", (long long)n);
-        // Measured on MI355X (c64 2^20 x 128): one launch per pass over the
-        // whole batch beats chunking through a small scratch ring (launch
-        // tails outweigh any Infinity-Cache reuse), so chunks only bound the
-        // scratch size.
-        const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
-        int64_t chunk = std::max<int64_t>(1, chunk_bytes / (n * (int64_t)csize));
+        if (opts.dual && !store) {
+            const int st = four_step_dual(n, N1, N2, batch, in_buf, in_off, in_dist, out_buf, out_off, out_dist, first,
+                                          last);
+            if (st != kNotApplicable) return st;
+        }
+        // Run one after the other, the two passes are each fastest as one
+        // launch over the whole batch: chunks only bound the scratch size
+        // (measured on MI355X, c64 2^20 x 128: sequential chunks through a
+        // small ring gain nothing, launch tails outweigh any Infinity-Cache reuse).
+        int64_t chunk = std::max<int64_t>(1, chunk_bytes() / (n * (int64_t)csize));
         chunk = std::min(chunk, batch);
         need_scratch(chunk * n);
         const int pm1 = opts.col_m, pm2 = opts.row_m;
@@ -674,6 +707,84 @@ struct Builder {
                       n, N2, 1, 0, out_dist, 1, N1, TW_NONE, 0, false, last);
             if (st) return st;
         }
+        return GFFT_OK;
+    }
+
+    static constexpr int kNotApplicable = -1;
+
+    // The overlapped four-step: the batch in chunks of dc transforms through a
+    // ring of two scratch slots; launch 0 runs the column pass of chunk 0,
+    // launch k (1 <= k < chunks) pairs the column pass of chunk k with the
+    // transposing row pass of chunk k - 1 (dual_pipe_kernel, the workgroups
+    // split between the two), and the last launch runs the row pass of the
+    // last chunk.  Launches on one stream run in order, so slot k & 1 is
+    // written by launch k and read by launch k + 1 only after every write of
+    // launch k: no wait inside a kernel.  Returns kNotApplicable when no dual
+    // kernel pairs the two passes of this split, or the batch is one chunk.
+    int four_step_dual(int64_t n, int64_t N1, int64_t N2, int64_t batch, int in_buf, int64_t in_off, int64_t in_dist,
+                       int out_buf, int64_t out_off, int64_t out_dist, bool first, bool last)
+    {
+        const int64_t bytes = std::min<int64_t>(env_int("GFFT_DUAL_CHUNK_BYTES", (int64_t)64 << 20), chunk_bytes() / 2);
+        const int64_t dc = std::max<int64_t>(1, bytes / (n * (int64_t)csize));
+        if (dc >= batch) return kNotApplicable;
+        const int pm1 = opts.col_m, pm2 = opts.row_m;
+        const KernelEntry *kc = find_tile(prec, (int)N1, PAT_COL, PAT_COL, pm1, opts.pipe_mask, -1, store);
+        const KernelEntry *kr = find_tile(prec, (int)N2, PAT_ROW, PAT_COL, pm2, opts.pipe_mask, -1, store);
+        const DualEntry *d = find_dual(prec, kc, kr);
+        if (!d) return kNotApplicable;
+        const int64_t chunks = (batch + dc - 1) / dc;
+        need_scratch(2 * dc * n);
+        auto take = [&](Launch &l) {
+            l = p->launches.back();
+            p->launches.pop_back();
+        };
+        auto col = [&](int64_t c, Launch &l) {
+            const int64_t cb = std::min(dc, batch - c * dc);
+            const int st = tile((int)N1, PAT_COL, PAT_COL, pm1, in_buf, in_off + c * dc * in_dist, B_SCR,
+                                (c & 1) * dc * n, N2, cb, 1, 0, in_dist, 1, N2, 0, n, 1, N2, TW_INNER, n, first, false);
+            if (!st) take(l);
+            return st;
+        };
+        auto row = [&](int64_t c, Launch &l) {
+            const int64_t cb = std::min(dc, batch - c * dc);
+            const int st = tile((int)N2, PAT_ROW, PAT_COL, pm2, B_SCR, (c & 1) * dc * n, out_buf,
+                                out_off + c * dc * out_dist, N1, cb, 1, 0, n, N2, 1, 0, out_dist, 1, N1, TW_NONE, 0,
+                                false, last);
+            if (!st) take(l);
+            return st;
+        };
+        int per_cu = 1;
+        if (!dry && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)d->fn, d->NT, 0) != hipSuccess)
+            per_cu = 1;
+        const int64_t total = (int64_t)std::max(1, per_cu) * std::max(1, p->num_cus);
+        // workgroups of the column role, in 256ths of the grid (measured with
+        // copies in the two passes' access patterns: an even split is best)
+        const int64_t share = std::min<int64_t>(255, std::max<int64_t>(1, env_int("GFFT_DUAL_SPLIT", 128)));
+        const int64_t ga0 = std::max<int64_t>(1, std::min(total - 1, total * share / 256));
+        Launch l, a, b;
+        int st = col(0, l);
+        if (st) return st;
+        p->launches.push_back(l);
+        for (int64_t c = 1; c < chunks; c++) {
+            if ((st = col(c, a)) || (st = row(c - 1, b))) return st;
+            a.type = LT_DUAL;
+            a.dual = d;
+            a.kb = b.k;
+            a.tb = b.ta;
+            a.in_buf_b = b.in_buf;
+            a.out_buf_b = b.out_buf;
+            a.in_off_b = b.in_off;
+            a.out_off_b = b.out_off;
+            a.vin_ok_b = b.vin_ok;
+            a.vout_ok_b = b.vout_ok;
+            a.first_b = b.first;
+            a.last_b = b.last;
+            a.split = std::min<int64_t>(ga0, a.ta.n_tiles);
+            a.grid = a.split + std::min<int64_t>(total - ga0, b.ta.n_tiles);
+            p->launches.push_back(a);
+        }
+        if ((st = row(chunks - 1, l))) return st;
+        p->launches.push_back(l);
         return GFFT_OK;
     }
 
@@ -895,12 +1006,12 @@ bool output_before_input_done(const std::vector<Launch> &launches)
 {
     size_t first_write = launches.size();
     for (size_t i = 0; i < launches.size(); i++)
-        if (launches[i].out_buf == B_OUT) {
+        if (launches[i].writes(B_OUT)) {
             first_write = i;
             break;
         }
     for (size_t i = first_write; i < launches.size(); i++)
-        if (launches[i].in_buf == B_IN) return true;
+        if (launches[i].reads(B_IN)) return true;
     return false;
 }
 
@@ -959,8 +1070,7 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         // unpack with an odd output distance needs the same; with an even
         // one it chunks the transform through at most GFFT_CHUNK_BYTES, so a
         // larger batch keeps that path.
-        const int64_t chunk_bytes = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
-        if (half && b.r2c_fusable(n) && b.opts.r2c_fuse && (od % 2 || batch * (n / 2) * (int64_t)cs <= chunk_bytes)) {
+        if (half && b.r2c_fusable(n) && b.opts.r2c_fuse && (od % 2 || batch * (n / 2) * (int64_t)cs <= b.chunk_bytes())) {
             st = b.r2c_fused(n, batch, id, od);
             break;
         }
@@ -1256,6 +1366,14 @@ int plan_via_dense_copy(gfft_plan_s *p, bool dry, const PlanOpts *forced, const 
             l.out_buf = B_SCR;
             l.out_off += region;
         }
+        if (l.type == LT_DUAL && l.in_buf_b == moved) {
+            l.in_buf_b = B_SCR;
+            l.in_off_b += region;
+        }
+        if (l.type == LT_DUAL && l.out_buf_b == moved) {
+            l.out_buf_b = B_SCR;
+            l.out_off_b += region;
+        }
         p->launches.push_back(l);
     }
     if (!input_side) p->launches.push_back(r);
@@ -1282,23 +1400,48 @@ int validate(const gfft_plan_s *p)
         return GFFT_OK;
     };
     int idx = 0;
+    // one tile pass: every sequence it loads and stores lies inside its buffers
+    auto tile_inside = [&](const KernelEntry *k, const TileArgs &a, int in_buf, int64_t in_off, int out_buf,
+                           int64_t out_off) {
+        const int L = k->L;
+        const int64_t n_outer2 = a.n_tiles / ((int64_t)a.tiles_inner * a.n_outer1);
+        if ((int64_t)a.tiles_inner * a.n_outer1 * n_outer2 != a.n_tiles)
+            return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d tile count", idx);
+        const int64_t li = (n_outer2 - 1) * a.in_outer2 + (int64_t)(a.n_outer1 - 1) * a.in_outer +
+                           (int64_t)(a.n_inner - 1) * a.in_inner + (int64_t)(L - 1) * a.in_elem;
+        const int64_t lo = (n_outer2 - 1) * a.out_outer2 + (int64_t)(a.n_outer1 - 1) * a.out_outer +
+                           (int64_t)(a.n_inner - 1) * a.out_inner + (int64_t)(L - 1) * a.out_elem;
+        int st;
+        if ((st = inside(in_buf, in_off, (li + 1) * cs, "tile load"))) return st;
+        if ((st = inside(out_buf, out_off, (lo + 1) * cs, "tile store"))) return st;
+        if (k->lds_bytes > 160 * 1024) return fail(GFFT_ERR_UNSUPPORTED, "internal: LDS");
+        return GFFT_OK;
+    };
     for (const Launch &l : p->launches) {
         int st = GFFT_OK;
         idx++;
         switch (l.type) {
-        case LT_TILE: {
-            const TileArgs &a = l.ta;
-            const int L = l.k->L;
-            const int64_t n_outer2 = a.n_tiles / ((int64_t)a.tiles_inner * a.n_outer1);
-            if ((int64_t)a.tiles_inner * a.n_outer1 * n_outer2 != a.n_tiles)
-                return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d tile count", idx);
-            const int64_t li = (n_outer2 - 1) * a.in_outer2 + (int64_t)(a.n_outer1 - 1) * a.in_outer +
-                               (int64_t)(a.n_inner - 1) * a.in_inner + (int64_t)(L - 1) * a.in_elem;
-            const int64_t lo = (n_outer2 - 1) * a.out_outer2 + (int64_t)(a.n_outer1 - 1) * a.out_outer +
-                               (int64_t)(a.n_inner - 1) * a.out_inner + (int64_t)(L - 1) * a.out_elem;
-            if ((st = inside(l.in_buf, l.in_off, (li + 1) * cs, "tile load"))) return st;
-            if ((st = inside(l.out_buf, l.out_off, (lo + 1) * cs, "tile store"))) return st;
-            if (l.k->lds_bytes > 160 * 1024) return fail(GFFT_ERR_UNSUPPORTED, "internal: LDS");
+        case LT_TILE:
+            if ((st = tile_inside(l.k, l.ta, l.in_buf, l.in_off, l.out_buf, l.out_off))) return st;
+            break;
+        case LT_DUAL: {
+            if ((st = tile_inside(l.k, l.ta, l.in_buf, l.in_off, l.out_buf, l.out_off))) return st;
+            if ((st = tile_inside(l.kb, l.tb, l.in_buf_b, l.in_off_b, l.out_buf_b, l.out_off_b))) return st;
+            if (!l.dual || l.dual->lds_bytes > 160 * 1024 || l.split < 1 || l.grid <= l.split ||
+                l.split > l.ta.n_tiles || l.grid - l.split > l.tb.n_tiles)
+                return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d dual split", idx);
+            // the two roles never touch the same bytes of one buffer
+            auto span = [&](const KernelEntry *k, const TileArgs &a, bool out, int64_t off) {
+                const int64_t n_outer2 = a.n_tiles / ((int64_t)a.tiles_inner * a.n_outer1);
+                const int64_t last = out ? (n_outer2 - 1) * a.out_outer2 + (int64_t)(a.n_outer1 - 1) * a.out_outer +
+                                               (int64_t)(a.n_inner - 1) * a.out_inner + (int64_t)(k->L - 1) * a.out_elem
+                                         : (n_outer2 - 1) * a.in_outer2 + (int64_t)(a.n_outer1 - 1) * a.in_outer +
+                                               (int64_t)(a.n_inner - 1) * a.in_inner + (int64_t)(k->L - 1) * a.in_elem;
+                return std::make_pair(off, off + (last + 1) * cs);
+            };
+            const auto aw = span(l.k, l.ta, true, l.out_off), br = span(l.kb, l.tb, false, l.in_off_b);
+            if (l.out_buf == l.in_buf_b && aw.first < br.second && br.first < aw.second)
+                return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d roles overlap", idx);
             break;
         }
         case LT_UNPACK: {
@@ -1377,6 +1520,9 @@ int validate(const gfft_plan_s *p)
                 return st;
             break;
         }
+        case LT_MIX:
+        case LT_MIX_RESET:
+            return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d belongs to batch plans", idx);
         }
     }
     return GFFT_OK;
@@ -1413,6 +1559,22 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
                      (long long)a.in_outer2, (long long)a.in_outer, (long long)a.in_inner, (long long)a.in_elem,
                      (long long)a.out_outer2, (long long)a.out_outer, (long long)a.out_inner,
                      (long long)a.out_elem, a.tw_mode, (int)l.first, (int)l.last, (int)l.vin_ok, (int)l.vout_ok, a.runs);
+            break;
+        }
+        case LT_DUAL: {
+            auto role = [&](const KernelEntry *k, const TileArgs &a, int ib, int64_t io, int ob, int64_t oo) {
+                char r[384];
+                snprintf(r, sizeof(r), "L=%d M=%d inp=%d outp=%d rad=%d,%d,%d,%d tiles=%d n_inner=%d tiles_inner=%d "
+                         "n_outer1=%d in=%s+%lld out=%s+%lld tw=%d runs=%d", k->L, k->M, k->inp, k->outp, k->rad[0],
+                         k->nrad > 1 ? k->rad[1] : 0, k->nrad > 2 ? k->rad[2] : 0, k->nrad > 3 ? k->rad[3] : 0,
+                         a.n_tiles, a.n_inner, a.tiles_inner, a.n_outer1, bn[ib], (long long)io, bn[ob], (long long)oo,
+                         a.tw_mode, a.runs);
+                return std::string(r);
+            };
+            snprintf(line, sizeof(line), "dual NT=%d lds=%d grid=%lld split=%lld | %s | %s\n", l.dual->NT,
+                     l.dual->lds_bytes, (long long)l.grid, (long long)l.split,
+                     role(l.k, l.ta, l.in_buf, l.in_off, l.out_buf, l.out_off).c_str(),
+                     role(l.kb, l.tb, l.in_buf_b, l.in_off_b, l.out_buf_b, l.out_off_b).c_str());
             break;
         }
         case LT_UNPACK:
@@ -1554,6 +1716,27 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             HIP_TRY(hipLaunchKernel((const void *)l.k->fn, dim3((unsigned)l.grid), dim3(l.k->NT), args, 0, s));
             break;
         }
+        case LT_DUAL: {
+            auto role = [&](const TileArgs &t, int ib, int64_t io, int ob, int64_t oo, bool vi, bool vo, bool first,
+                            bool last) {
+                TileArgs a = t;
+                a.in = bufs[ib] + io;
+                a.out = bufs[ob] + oo;
+                a.swap_in = inv && first;
+                a.swap_out = inv && last;
+                a.vec_in = vi && (((uintptr_t)a.in & 15) == 0);
+                a.vec_out = vo && (((uintptr_t)a.out & 15) == 0);
+                a.xcd_remap = p->xcd_remap && a.runs >= 0;
+                return a;
+            };
+            TileArgs a = role(l.ta, l.in_buf, l.in_off, l.out_buf, l.out_off, l.vin_ok, l.vout_ok, l.first, l.last);
+            TileArgs b = role(l.tb, l.in_buf_b, l.in_off_b, l.out_buf_b, l.out_off_b, l.vin_ok_b, l.vout_ok_b,
+                              l.first_b, l.last_b);
+            int split = (int)l.split;
+            void *args[] = {&a, &b, &split};
+            HIP_TRY(hipLaunchKernel((const void *)l.dual->fn, dim3((unsigned)l.grid), dim3(l.dual->NT), args, 0, s));
+            break;
+        }
         case LT_UNPACK:
         case LT_PACK: {
             UnpackArgs a = l.ua;
@@ -1644,6 +1827,7 @@ int on_plan_device(int plan_device)
         return fail(GFFT_ERR_INVALID_ARG, "plan belongs to device %d but device %d is current", plan_device, cur);
     return GFFT_OK;
 }
+
 
 
 // ---------------------------------------------------------------------------
@@ -1783,28 +1967,37 @@ int build_batch(gfft_batch_plan_s *p)
         if (in_side) l.vin_ok = l.vin_ok && p->offsets_even;
         if (out_side) l.vout_ok = l.vout_ok && p->offsets_even;
     };
+    int cnt = 0;
+    const KernelEntry *tab = tile_table(prec, 0, &cnt);
     for (const Cls &c : big) {
         const int64_t L = (int64_t)1 << c.lg;
+        // each length class takes the kernel choices a single plan of that
+        // length would: the tuning table's entry, or the defaults
+        b.opts = resolve_opts(prec, L, nullptr);
         if (L <= maxrow) {
+            // one transform per tile (job tables give each its own offsets): a
+            // tuned single-pass kernel counts only when it holds one sequence
+            const int v = b.opts.row_variant;
+            const int variant = v >= 0 && v < cnt && tab[v].M == 1 ? v : -1;
             if ((st = b.tile((int)L, PAT_ROW, PAT_ROW, 1, B_IN, 0, B_OUT, 0, 1, c.count, 1, 0, 0, 0, 1, 0, 0, 0, 1,
-                             TW_NONE, 0, false, false)))
+                             TW_NONE, 0, false, false, variant)))
                 return st;
             set_jobs(c.first, true, true);
             continue;
         }
         int l1, l2;
-        four_step_split(prec, c.lg, &l1, &l2);
+        four_step_split(prec, c.lg, &l1, &l2, b.opts.n1_log2);
         const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
         const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(c.count, ((int64_t)1 << 30) / (L * (int64_t)cs)));
         b.need_scratch(chunk * L);
         for (int64_t b0 = 0; b0 < c.count; b0 += chunk) {
             const int64_t cb = std::min(chunk, c.count - b0);
-            if ((st = b.tile((int)N1, PAT_COL, PAT_COL, 0, B_IN, 0, B_SCR, 0, N2, cb, 1, 0, 0, 1, N2, 0, L, 1, N2,
-                             TW_INNER, L, false, false)))
+            if ((st = b.tile((int)N1, PAT_COL, PAT_COL, b.opts.col_m, B_IN, 0, B_SCR, 0, N2, cb, 1, 0, 0, 1, N2, 0, L,
+                             1, N2, TW_INNER, L, false, false)))
                 return st;
             set_jobs(c.first + b0, true, false);
-            if ((st = b.tile((int)N2, PAT_ROW, PAT_COL, 0, B_SCR, 0, B_OUT, 0, N1, cb, 1, 0, L, N2, 1, 0, 0, 1, N1,
-                             TW_NONE, 0, false, false)))
+            if ((st = b.tile((int)N2, PAT_ROW, PAT_COL, b.opts.row_m, B_SCR, 0, B_OUT, 0, N1, cb, 1, 0, L, N2, 1, 0, 0,
+                             1, N1, TW_NONE, 0, false, false)))
                 return st;
             set_jobs(c.first + b0, false, true);
         }
@@ -1832,17 +2025,19 @@ int tuning_load_locked(const char *path)
     char line[1024];
     int loaded = 0;
     while (fgets(line, sizeof(line), f)) {
-        int prec, a, b, c, d, e;
+        int prec, a, b, c, d, e, du = 0;
         long long n;
-        if (sscanf(line, " {\"precision\": %d, \"n\": %lld, \"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, "
-                         "\"pipe_mask\": %d, \"row_variant\": %d",
-                   &prec, &n, &a, &b, &c, &d, &e) == 7) {
+        const int got = sscanf(line, " {\"precision\": %d, \"n\": %lld, \"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, "
+                                     "\"pipe_mask\": %d, \"row_variant\": %d, \"dual\": %d",
+                               &prec, &n, &a, &b, &c, &d, &e, &du);
+        if (got >= 7) {  // files written before the dual field read as dual = 0
             PlanOpts o;
             o.n1_log2 = a;
             o.col_m = b;
             o.row_m = c;
             o.pipe_mask = d;
             o.row_variant = e;
+            o.dual = got == 8 ? du : 0;
             g_tune[{prec, (int64_t)n}] = o;
             loaded++;
         }
@@ -1895,6 +2090,12 @@ std::vector<PlanOpts> tune_candidates(int prec, int64_t n)
                     o.row_m = rm;
                     o.pipe_mask = pm;
                     out.push_back(o);
+                    // the overlapped form where a dual kernel pairs the two passes
+                    if (pm == 3 && find_dual(prec, find_tile(prec, (int)N1, PAT_COL, PAT_COL, cm, 3),
+                                             find_tile(prec, (int)N2, PAT_ROW, PAT_COL, rm, 3))) {
+                        o.dual = 1;
+                        out.push_back(o);
+                    }
                 }
     }
     return out;
@@ -1962,8 +2163,8 @@ int gfft_autotune(int precision, int64_t n, int64_t batch, int reps, char *repor
         char line[256];
         snprintf(line, sizeof(line),
                  "  {\"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, \"pipe_mask\": %d, \"row_variant\": %d, "
-                 "\"ms\": %.5f}",
-                 o.n1_log2, o.col_m, o.row_m, o.pipe_mask, o.row_variant, ms);
+                 "\"dual\": %d, \"ms\": %.5f}",
+                 o.n1_log2, o.col_m, o.row_m, o.pipe_mask, o.row_variant, o.dual, ms);
         rep += (tried ? ",\n" : "") + std::string(line);
         tried++;
         if (ms < best_ms) {
@@ -1986,8 +2187,8 @@ int gfft_autotune(int precision, int64_t n, int64_t batch, int reps, char *repor
     char tail[256];
     snprintf(tail, sizeof(tail),
              "\n], \"best\": {\"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, \"pipe_mask\": %d, \"row_variant\": %d, "
-             "\"ms\": %.5f}}",
-             best.n1_log2, best.col_m, best.row_m, best.pipe_mask, best.row_variant, best_ms);
+             "\"dual\": %d, \"ms\": %.5f}}",
+             best.n1_log2, best.col_m, best.row_m, best.pipe_mask, best.row_variant, best.dual, best_ms);
     rep += tail;
     if (report && len) snprintf(report, len, "%s", rep.c_str());
     return GFFT_OK;
@@ -2005,9 +2206,9 @@ int gfft_tuning_save(const char *path)
         const PlanOpts &o = kv.second;
         fprintf(f,
                 " {\"precision\": %d, \"n\": %lld, \"n1_log2\": %d, \"col_m\": %d, \"row_m\": %d, \"pipe_mask\": %d, "
-                "\"row_variant\": %d}%s\n",
+                "\"row_variant\": %d, \"dual\": %d}%s\n",
                 kv.first.first, (long long)kv.first.second, o.n1_log2, o.col_m, o.row_m, o.pipe_mask, o.row_variant,
-                ++i < g_tune.size() ? "," : "");
+                o.dual, ++i < g_tune.size() ? "," : "");
     }
     fprintf(f, "]}\n");
     fclose(f);
@@ -2251,12 +2452,21 @@ int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)
 {
     if (!plan) return fail(GFFT_ERR_INVALID_ARG, "null plan");
     if (!in || !out) return fail(GFFT_ERR_INVALID_ARG, "null buffer");
+    if (int st = on_plan_device(plan->device)) return st;
     std::lock_guard<std::mutex> lk(plan->host_mu);
-    if (!plan->hin) {
-        if (hipMalloc(&plan->hin, plan->in_bytes) != hipSuccess)
+    if (!plan->hin || !plan->hout) {
+        // both staging buffers or neither: a failed second allocation rolls back the first
+        void *hin = nullptr, *hout = nullptr;
+        if (hipMalloc(&hin, plan->in_bytes) != hipSuccess)
             return fail(GFFT_ERR_ALLOC, "device staging of %zu bytes", plan->in_bytes);
-        if (hipMalloc(&plan->hout, plan->out_bytes) != hipSuccess)
+        if (hipMalloc(&hout, plan->out_bytes) != hipSuccess) {
+            (void)hipFree(hin);
             return fail(GFFT_ERR_ALLOC, "device staging of %zu bytes", plan->out_bytes);
+        }
+        if (plan->hin) (void)hipFree(plan->hin);
+        if (plan->hout) (void)hipFree(plan->hout);
+        plan->hin = hin;
+        plan->hout = hout;
     }
     HIP_TRY(hipMemcpy(plan->hin, in, plan->in_bytes, hipMemcpyHostToDevice));
     // keep the bytes the transform does not write (pitch gaps) intact

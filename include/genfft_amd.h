@@ -142,6 +142,7 @@ GFFT_API int gfft_plan_destroy(gfft_plan plan);
  * for the plan's previous work, because they share its workspace. */
 GFFT_API int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void *stream);
 
+
 /* gfft_execute for 16-bit storage plans with a per-transform scale:
  * transform b (one batch member: a 1D transform, a 2D image, a COLS matrix)
  * is stored as scales[b] * FFT(x_b), the factor applied in FP32 before the

@@ -228,13 +228,16 @@ def test_python_wrappers_reject_wrong_types_and_short_buffers(gpu):
     with pytest.raises(TypeError):
         p.execute(ok, torch.zeros(4 * n, dtype=torch.complex128, device=gpu))
     with pytest.raises(ValueError):
-        p.execute(ok[: 3 * n], ok)
+        p.execute(torch.zeros(3 * n, dtype=torch.complex64, device=gpu), ok)
     with pytest.raises(ValueError):
         p.execute(ok, torch.zeros(4 * n - 1, dtype=torch.complex64, device=gpu))
     r = G.Plan(C.GFFT_R2C, n, batch=2, half=True)
-    with pytest.raises(TypeError):
-        r.execute(torch.zeros(2 * n, dtype=torch.complex64, device=gpu),
+    with pytest.raises(TypeError):  # a real side takes its reals, or its complex dtype (in place)
+        r.execute(torch.zeros(2 * n, dtype=torch.float64, device=gpu),
                   torch.zeros(2 * (n // 2 + 1), dtype=torch.complex64, device=gpu))
+    with pytest.raises(TypeError):  # the complex side must be complex
+        r.execute(torch.zeros(2 * n, dtype=torch.float32, device=gpu),
+                  torch.zeros(4 * (n // 2 + 1), dtype=torch.float32, device=gpu))
     with pytest.raises(ValueError):
         G.irfft(torch.zeros(2, n // 2, dtype=torch.complex64, device=gpu), n)
     with pytest.raises(TypeError):

@@ -100,3 +100,29 @@ def test_uneven_in_place_and_large_classes(gpu):
     t = torch.from_numpy(x).to(gpu)
     b.execute(t, t, mode="graph")   # in place
     check_jobs(x, t.cpu().numpy(), sizes, offs, offs, inv, "f32", range(len(sizes)))
+
+
+def test_uneven_full_configuration_sample(gpu):
+    """The benchmark configuration itself: 20,000 complex64 transforms of
+    2^k, k uniform in 6..16 (seed 0), entries U[-1, 1), 50/50 directions, run
+    as the captured graph (replayed twice); 500 random jobs against the
+    long-double oracle, every class represented."""
+    import genfft_amd as G
+    rng = np.random.default_rng(0)
+    count = 20000
+    sizes = (1 << rng.integers(6, 17, count)).astype(np.int64)
+    inv = rng.integers(0, 2, count).astype(bool)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    total = int(sizes.sum())
+    g = torch.Generator(device=gpu).manual_seed(0)
+    x = torch.complex(torch.rand(total, device=gpu, generator=g) * 2 - 1,
+                      torch.rand(total, device=gpu, generator=g) * 2 - 1)
+    y = torch.empty_like(x)
+    b = G.UnevenBatch(sizes, offs, offs, inv, torch.complex64)
+    b.execute(x, y, mode="graph")
+    b.execute(x, y, mode="graph")
+    torch.cuda.synchronize()
+    xs, ys = x.cpu().numpy(), y.cpu().numpy()
+    pick = np.random.default_rng(1).choice(count, 500, replace=False)
+    assert len(set(int(sizes[j]) for j in pick)) == 11
+    check_jobs(xs, ys, sizes, offs, offs, inv, "f32", pick)
