@@ -37,6 +37,8 @@ GFFT_BF16 = 3
 GFFT_SCALE_GIVEN = 0
 GFFT_SCALE_AUTO = 1
 GFFT_FLAG_HALF = 1
+GFFT_FLAG_NO_WORKSPACE = 2
+GFFT_NOALLOC_SCRATCH_BYTES = 64 << 20
 GFFT_FORWARD = 0
 GFFT_INVERSE = 1
 
@@ -45,6 +47,7 @@ EXPORTS = (
     "gfft_plan_create",
     "gfft_plan_destroy",
     "gfft_execute",
+    "gfft_execute_noalloc",
     "gfft_execute_host",
     "gfft_execute_scaled",
     "gfft_execute_scrambled",
@@ -129,6 +132,8 @@ def lib() -> ctypes.CDLL:
         L.gfft_plan_destroy.restype = i32
         L.gfft_execute.argtypes = [vp, vp, vp, i32, vp]
         L.gfft_execute.restype = i32
+        L.gfft_execute_noalloc.argtypes = [vp, vp, vp, i32, vp]
+        L.gfft_execute_noalloc.restype = i32
         L.gfft_execute_host.argtypes = [vp, vp, vp, i32]
         L.gfft_execute_host.restype = i32
         L.gfft_plan_workspace.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]

@@ -44,6 +44,13 @@
 #include "gfft_r2c.h"
 #include "gfft_registry.h"
 
+// Workspace of gfft_execute_noalloc: one static device array, instantiated
+// on every GPU the library's code object is loaded on and held until the
+// process ends.  Host code must never pass this name to a kernel: on the host
+// it is only a stand-in the runtime resolves to each device's own copy
+// (noalloc_scratch_ptr below).
+static __device__ __align__(256) unsigned char noalloc_scratch[GFFT_NOALLOC_SCRATCH_BYTES];
+
 namespace gfft {
 namespace {
 
@@ -555,8 +562,14 @@ struct Builder {
     void need_scratch(int64_t elems) { scratch_elems = std::max(scratch_elems, elems); }
 
     // Largest batch chunk a four-step plan runs through its workspace: 1 GiB
-    // (GFFT_CHUNK_BYTES overrides).
-    int64_t chunk_bytes() const { return env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30); }
+    // (GFFT_CHUNK_BYTES overrides), or the static workspace of
+    // gfft_execute_noalloc for plans created without a workspace.
+    int64_t chunk_bytes() const
+    {
+        int64_t c = env_int("GFFT_CHUNK_BYTES", (int64_t)1 << 30);
+        if (p->d.flags & GFFT_FLAG_NO_WORKSPACE) c = std::min<int64_t>(c, GFFT_NOALLOC_SCRATCH_BYTES);
+        return c;
+    }
 
     // one tile pass
     int tile(int L, int inp, int outp, int prefer_m, int in_buf, int64_t in_off_el, int out_buf,
@@ -1653,6 +1666,9 @@ bool needs_stage(const gfft_plan_s *p, const void *in, const void *out)
 int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
         hipEvent_t *events = nullptr, const float *scale = nullptr)
 {
+    if (p->scratch_bytes && !p->scratch)
+        return fail(GFFT_ERR_INVALID_ARG, "plan created with GFFT_FLAG_NO_WORKSPACE holds no workspace: "
+                                          "run it with gfft_execute_noalloc");
     const bool stage = needs_stage(p, in, out);
     if (!p->scratch && !stage)  // no shared workspace: calls on any streams are independent
         return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events, scale,
@@ -1828,6 +1844,40 @@ int on_plan_device(int plan_device)
     return GFFT_OK;
 }
 
+// Device address of noalloc_scratch on device `dev`, which must be current:
+// resolved once per device (each GPU holds its own copy of the array at its
+// own address) and kept.  Kernels and copies take this address, never the
+// array's host-side name.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch_ptr(int dev, void **ptr)
+{
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(GFFT_ERR_UNSUPPORTED, "device %d beyond the %d the static workspace table covers", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *a = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess || !a)
+            return fail(GFFT_ERR_HIP, "hipGetSymbolAddress(noalloc_scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_noalloc_ptr[dev] = a;
+    }
+    *ptr = g_noalloc_ptr[dev];
+    return GFFT_OK;
+}
+
+// A plan created without a workspace must fit the static one.
+int check_noalloc_fit(const gfft_plan_s *p)
+{
+    if ((p->d.flags & GFFT_FLAG_NO_WORKSPACE) && p->scratch_bytes > (size_t)GFFT_NOALLOC_SCRATCH_BYTES)
+        return fail(GFFT_ERR_UNSUPPORTED,
+                    "this plan needs %zu bytes of workspace, more than the %lld bytes of the static workspace "
+                    "(GFFT_FLAG_NO_WORKSPACE / gfft_execute_noalloc)",
+                    p->scratch_bytes, (long long)GFFT_NOALLOC_SCRATCH_BYTES);
+    return GFFT_OK;
+}
 
 
 // ---------------------------------------------------------------------------
@@ -2321,6 +2371,7 @@ int gfft_plan_explain(const gfft_desc *desc, char *buf, size_t len)
     set_precision(p.get(), desc->precision);
     if ((st = build_plan(p.get(), true))) return st;
     if ((st = validate(p.get()))) return st;
+    if ((st = check_noalloc_fit(p.get()))) return st;
     return describe(p.get(), buf, len);
 }
 
@@ -2342,7 +2393,13 @@ int gfft_plan_create(gfft_plan *plan, const gfft_desc *desc)
         p->num_cus = 256;
     if ((st = build_plan(p.get(), false))) return st;
     if ((st = validate(p.get()))) return st;
-    if (p->scratch_bytes) {
+    if (d.flags & GFFT_FLAG_NO_WORKSPACE) {
+        // set-up for gfft_execute_noalloc on this device: resolve the static
+        // workspace's address now, so the calls themselves touch no runtime table
+        void *scr = nullptr;
+        if ((st = check_noalloc_fit(p.get()))) return st;
+        if (p->scratch_bytes && (st = noalloc_scratch_ptr(dev, &scr))) return st;
+    } else if (p->scratch_bytes) {
         hipError_t e = hipMalloc(&p->scratch, p->scratch_bytes);
         if (e != hipSuccess)
             return fail(GFFT_ERR_ALLOC, "scratch of %zu bytes: %s", p->scratch_bytes, hipGetErrorString(e));
@@ -2374,6 +2431,30 @@ int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void 
                     plan->dir_mask == 1 ? "forward" : "inverse");
     if (int st = on_plan_device(plan->device)) return st;
     return run(plan, in, out, direction, (hipStream_t)stream);
+}
+
+int gfft_execute_noalloc(gfft_plan plan, const void *in, void *out, int direction, void *stream)
+{
+    if (!plan) return fail(GFFT_ERR_INVALID_ARG, "null plan");
+    if (!in || !out) return fail(GFFT_ERR_INVALID_ARG, "null buffer");
+    if (direction != GFFT_FORWARD && direction != GFFT_INVERSE)
+        return fail(GFFT_ERR_INVALID_ARG, "direction must be GFFT_FORWARD or GFFT_INVERSE");
+    if (!(plan->dir_mask & (1 << direction)))
+        return fail(GFFT_ERR_INVALID_ARG, "this plan kind supports only the %s direction",
+                    plan->dir_mask == 1 ? "forward" : "inverse");
+    if (int st = on_plan_device(plan->device)) return st;
+    if (plan->scratch_bytes > (size_t)GFFT_NOALLOC_SCRATCH_BYTES)
+        return fail(GFFT_ERR_UNSUPPORTED,
+                    "this plan needs %zu bytes of workspace, more than the %lld bytes of the static workspace "
+                    "(create it with GFFT_FLAG_NO_WORKSPACE to chunk its batch)",
+                    plan->scratch_bytes, (long long)GFFT_NOALLOC_SCRATCH_BYTES);
+    if (needs_stage(plan, in, out))
+        return fail(GFFT_ERR_UNSUPPORTED, "the buffers overlap other than exactly in place, which needs a staging "
+                                          "copy this entry point cannot allocate");
+    void *scr = nullptr;
+    if (plan->scratch_bytes)
+        if (int st = noalloc_scratch_ptr(plan->device, &scr)) return st;
+    return run_launches(plan->launches, plan->prec, plan->xcd_remap, in, out, scr, direction, (hipStream_t)stream);
 }
 
 int gfft_execute_scaled(gfft_plan plan, const void *in, void *out, int direction, float *scales, int scale_mode,
@@ -2417,6 +2498,9 @@ int gfft_execute_scrambled(gfft_plan plan, const void *in, void *out, int direct
     if ((d.kind != GFFT_C2C && d.kind != GFFT_COLS) || plan->store)
         return fail(GFFT_ERR_UNSUPPORTED, "scrambled input needs a complex64 / complex128 C2C or COLS plan");
     if (int st = on_plan_device(plan->device)) return st;
+    if (plan->scratch_bytes && !plan->scratch)
+        return fail(GFFT_ERR_INVALID_ARG, "plan created with GFFT_FLAG_NO_WORKSPACE holds no workspace: "
+                                          "run it with gfft_execute_noalloc");
     BitrevArgs a{};
     a.n = d.n;
     a.lg = ilog2(d.n);

@@ -90,6 +90,14 @@ extern "C" {
 
 /* flags */
 #define GFFT_FLAG_HALF 1 /* R2C / DIT: write n/2+1 bins (else all n, Hermitian-filled) */
+/* plan_create allocates no workspace: the plan runs only through
+ * gfft_execute_noalloc, with the library's static workspace (batches are cut
+ * into chunks that fit it); plans needing more than
+ * GFFT_NOALLOC_SCRATCH_BYTES are rejected with GFFT_ERR_UNSUPPORTED */
+#define GFFT_FLAG_NO_WORKSPACE 2
+
+/* Size of the static device workspace gfft_execute_noalloc uses, per GPU. */
+#define GFFT_NOALLOC_SCRATCH_BYTES (64ll << 20)
 
 /* directions */
 #define GFFT_FORWARD 0
@@ -142,6 +150,22 @@ GFFT_API int gfft_plan_destroy(gfft_plan plan);
  * for the plan's previous work, because they share its workspace. */
 GFFT_API int gfft_execute(gfft_plan plan, const void *in, void *out, int direction, void *stream);
 
+/* gfft_execute that allocates nothing (no device memory, no events): for
+ * callers that cannot allocate at call time -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  No reference
+ * counterpart (genFFT allocates nothing on the device at all).  The plan's
+ * workspace is replaced by one static device array of
+ * GFFT_NOALLOC_SCRATCH_BYTES per GPU (held from the library's first use on
+ * that GPU until the process ends); a plan needing more, or a call whose
+ * buffers overlap other than exactly in place where the plan would stage its
+ * input, returns GFFT_ERR_UNSUPPORTED.  Create the plan with
+ * GFFT_FLAG_NO_WORKSPACE to keep its batches chunked to that size and to skip
+ * its own workspace (any plan that fits may be run).  Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU (other streams,
+ * other threads) race and must be ordered by the caller.  Nothing in the
+ * array needs clearing between calls (each launch writes what it later
+ * reads). */
+GFFT_API int gfft_execute_noalloc(gfft_plan plan, const void *in, void *out, int direction, void *stream);
 
 /* gfft_execute for 16-bit storage plans with a per-transform scale:
  * transform b (one batch member: a 1D transform, a 2D image, a COLS matrix)

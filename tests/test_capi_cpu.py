@@ -508,3 +508,23 @@ def test_dit_plans_for_multiples_of_four(prec):
         assert explain(desc(kind=C.GFFT_DIT, prec=prec, n=n))[0] == C.GFFT_ERR_UNSUPPORTED
     assert explain(desc(kind=C.GFFT_C2C, prec=prec, n=12))[0] == C.GFFT_ERR_UNSUPPORTED
     assert explain(desc(kind=C.GFFT_R2C, prec=prec, n=12))[0] == C.GFFT_ERR_UNSUPPORTED
+
+
+def test_workspace_less_plans_fit_the_static_workspace():
+    """GFFT_FLAG_NO_WORKSPACE plans (run through gfft_execute_noalloc) cut a
+    four-step batch into chunks that fit the static workspace; a plan that
+    cannot fit it is refused with a clear error."""
+    st, text = explain(desc(n=1 << 20, batch=128, flags=C.GFFT_FLAG_NO_WORKSPACE))
+    assert st == C.GFFT_OK, C.lib().gfft_last_error()
+    scratch = int(text.split("scratch_bytes=")[1].split()[0])
+    assert 0 < scratch <= C.GFFT_NOALLOC_SCRATCH_BYTES
+    assert text.count("\ntile") == 2 * 128 // (C.GFFT_NOALLOC_SCRATCH_BYTES // ((1 << 20) * 8))
+    # the same descriptor with its own workspace runs the batch in one chunk
+    st, text = explain(desc(n=1 << 20, batch=128))
+    assert st == C.GFFT_OK and text.count("\ntile") == 2
+    # an 8192 x 8192 2D plane needs its column sub-pass intermediate whole
+    st, _ = explain(desc(kind=C.GFFT_C2C_2D, n=8192, n2=8192, flags=C.GFFT_FLAG_NO_WORKSPACE))
+    assert st == C.GFFT_ERR_UNSUPPORTED and b"static workspace" in C.lib().gfft_last_error()
+    # single-pass plans need no workspace at all
+    st, text = explain(desc(n=1024, batch=4096, flags=C.GFFT_FLAG_NO_WORKSPACE))
+    assert st == C.GFFT_OK and "scratch_bytes=0" in text
