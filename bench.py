@@ -47,6 +47,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 WORKLOADS = {
     # name: (precision, n, batch)
     "c64_n2^20_b128": ("c64", 1 << 20, 128),
@@ -445,13 +450,17 @@ def bench_reference_cases(dev):
            "FFT_1D": [], "RealFFT_1D": [], "FFT_Vert": []}
     for k in range(1, 22):
         n = 1 << k
+        if k % 5 == 1:
+            log(f"reference cases: FFT_1D / RealFFT_1D n = 2^{k}")
         x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex64)
         h, d, err = case(C.GFFT_C2C, n, 0, x, np.empty_like(x), inv_plan=True)
         out["FFT_1D"].append({"n": n, "host_us": h, "device_us": d, "err": err})
         r = rng.uniform(-1, 1, n).astype(np.float32)
         h, d, _ = case(C.GFFT_R2C, n, 0, r, np.empty(n // 2 + 1, np.complex64))
         out["RealFFT_1D"].append({"n": n, "host_us": h, "device_us": d})
-    for nfft, ncols in _vert_args():
+    for i, (nfft, ncols) in enumerate(_vert_args()):
+        if i % 20 == 0:
+            log(f"reference cases: FFT_Vert {nfft} x {ncols}")
         x = (rng.uniform(-1, 1, nfft * ncols) + 1j * rng.uniform(-1, 1, nfft * ncols)).astype(np.complex64)
         h, d, err = case(C.GFFT_COLS, nfft, ncols, x, np.empty_like(x), inv_plan=True)
         out["FFT_Vert"].append({"nfft": nfft, "ncols": ncols, "host_us": h, "device_us": d, "err": err})
@@ -698,15 +707,16 @@ def main():
         acc_line = accuracy("c2c", x, y, n, n2, prec)
     cpu = None
     if not a.no_cpu and not half and kind == "c2c":
+        log("cpu baseline")
         cpu = cpu_baseline(x.cpu().numpy(), n, batch, prec, y.cpu().numpy())
     del x, y, plan
     torch.cuda.empty_cache()
     extras = {}
     if not a.no_extras and a.workload == "c64_n2^20_b128":
-        extras["fused_filter"] = bench_filter(dev)
-        extras["uneven_batch"] = bench_uneven(dev)
-        extras["precision"] = bench_precision(dev)
-        extras["reference_cases"] = bench_reference_cases(dev)
+        for name, leg in (("fused_filter", bench_filter), ("uneven_batch", bench_uneven),
+                          ("precision", bench_precision), ("reference_cases", bench_reference_cases)):
+            log(f"leg {name}")
+            extras[name] = leg(dev)
 
     line = {
         "metric": "batched c2c fp32 FFT GFLOP/s & %HBM peak at N=2^20; fused fwd+adjoint speedup",

@@ -13,7 +13,8 @@ struct FilterArgs {
     const void *H;        // n/2 + 1 complex bins
     void *partial;        // adjoint: per-row gH contributions, batch x (n/2 + 1)
     const void *tw;       // W_n^k, k < n/2 (complex T, FP64-generated, rounded once)
-    const void *pass_tw;  // Stockham twiddles of the n/2-point FFT
+    const void *pass_tw;  // Stockham twiddles of the n/2-point forward FFT
+    const void *pass_tw_inv;  // the same for the inverse FFT's radix order
     int64_t batch;
     int64_t bins;         // n/2 + 1
     int32_t conj_h;       // forward kernel with conj(H) (the x-adjoint alone)
@@ -28,7 +29,8 @@ struct ReduceArgs {
 
 // which: 0 forward, 1 adjoint (gx + partials), 2 partial reduction (gH)
 hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s);
-// threads and radices of the fused kernel for L = n/2 (0 if unsupported)
-int filter_shape(int prec, int L, int *nt, int *rad);
+// threads and radices (of the forward FFT, or with `inverse` of the inverse
+// FFT) of the fused kernel for L = n/2; 0 if unsupported
+int filter_shape(int prec, int L, int *nt, int *rad, int inverse);
 
 }  // namespace gfft

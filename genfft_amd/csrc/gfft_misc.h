@@ -61,15 +61,18 @@ struct CopyArgs {
     int32_t real_in, real_out;
 };
 
-// Rows of `width` reals copied between two layouts: row r of image b starts
-// at b * in_dist + r * in_pitch in the input and b * out_dist + r * out_pitch
-// in the output (real elements).  Moves a real side whose pitch or distance
-// is odd to or from a dense copy the tile kernels can read as complex pairs.
+// Rows of `width` elements of elem_bytes (4 or 8) copied between two layouts:
+// row r of image b starts at b * in_dist + r * in_pitch in the input and
+// b * out_dist + r * out_pitch in the output (in elements).  Moves a real side
+// whose pitch or distance is odd to or from a dense copy the tile kernels
+// can read as complex pairs, and runs the length-1 column transform (a copy).
 struct RepackArgs {
     const void *in;
     void *out;
     int64_t width, rows, batch;
     int64_t in_pitch, in_dist, out_pitch, out_dist;
+    int32_t elem_bytes;
+    int32_t pad;
 };
 
 // Per-transform scale of a 16-bit storage transform: scale[b] = 2^-k, the
@@ -106,6 +109,6 @@ hipError_t launch_separate(int prec, const SeparateArgs &a, hipStream_t s);
 hipError_t launch_interleave(int prec, const InterleaveArgs &a, hipStream_t s);
 hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s);
 hipError_t launch_copy(int prec, const CopyArgs &a, hipStream_t s);
-hipError_t launch_repack(int prec, const RepackArgs &a, hipStream_t s);
+hipError_t launch_repack(const RepackArgs &a, hipStream_t s);
 
 }  // namespace gfft

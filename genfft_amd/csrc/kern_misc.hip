@@ -9,8 +9,9 @@
 //   interleave  transform_interleave's packing (include/genFFT/fft.h:100-105).
 //   hfill       RealFFT2D's Hermitian column fill (include/genFFT/FFTReal.h:90-103).
 //   copy        the n = 1 transform (identity on each element).
-//   repack      rows of reals between two pitches (odd real-side layouts to
-//               and from a dense copy in scratch).
+//   repack      rows of 4- or 8-byte elements between two pitches (odd
+//               real-side layouts to and from a dense copy in scratch; the
+//               length-1 column transform).
 //   amax        per-transform power-of-two scale of 16-bit storage input.
 //   bitrev      bit-reversal gather of rows (transform_no_scramble input).
 //
@@ -386,11 +387,12 @@ hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_repack(int prec, const RepackArgs &a, hipStream_t s)
+hipError_t launch_repack(const RepackArgs &a, hipStream_t s)
 {
     const unsigned g = grid_for(a.width * a.rows * a.batch);
-    if (prec) hipLaunchKernelGGL(repack_kernel<double>, dim3(g), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(repack_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    if (a.elem_bytes == 8) hipLaunchKernelGGL(repack_kernel<double>, dim3(g), dim3(256), 0, s, a);
+    else if (a.elem_bytes == 4) hipLaunchKernelGGL(repack_kernel<float>, dim3(g), dim3(256), 0, s, a);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

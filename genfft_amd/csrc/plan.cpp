@@ -808,19 +808,16 @@ struct Builder {
     {
         if (rows == 1) {
             if (in_buf == out_buf && in_pitch == out_pitch && in_dist == out_dist) return GFFT_OK;
-            // identity copy of the rows: run as length-1 "rows" of a copy
-            for (int64_t c = 0; c < cols; c++) {
-                Launch l;
-                l.type = LT_COPY;
-                l.ca.batch = batch;
-                l.ca.in_dist = in_dist;
-                l.ca.out_dist = out_dist;
-                l.in_buf = in_buf;
-                l.out_buf = out_buf;
-                l.in_off = c * (int64_t)csize;
-                l.out_off = c * (int64_t)csize;
-                p->launches.push_back(l);
-            }
+            // the length-1 transform is the identity: one strided copy of the
+            // `cols` elements of every matrix, moved as 4- or 8-byte words
+            const int32_t word = csize == 4 ? 4 : (int32_t)(csize / 2);
+            const int64_t per = (int64_t)csize / word;
+            Launch l;
+            l.type = LT_REPACK;
+            l.rp = RepackArgs{nullptr, nullptr, cols * per, 1, batch, 0, in_dist * per, 0, out_dist * per, word, 0};
+            l.in_buf = in_buf;
+            l.out_buf = out_buf;
+            p->launches.push_back(l);
             return GFFT_OK;
         }
         const int maxcol = max_len(prec, PAT_COL, PAT_COL, store);
@@ -1073,7 +1070,7 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         if (id % 2) {
             gfft_desc dense = d;
             dense.in_dist = 0;
-            return plan_via_dense_copy(p, dry, forced, dense, true, RepackArgs{nullptr, nullptr, n, batch, 1, id, 0, n, 0});
+            return plan_via_dense_copy(p, dry, forced, dense, true, RepackArgs{nullptr, nullptr, n, batch, 1, id, 0, n, 0, (int32_t)rs, 0});
         }
         if (b.rows_fusable(n)) {
             st = b.real_rows(RR_R2C, n, batch, B_IN, 0, id / 2, B_OUT, 0, od, half);
@@ -1130,7 +1127,7 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         if (od % 2) {
             gfft_desc dense = d;
             dense.out_dist = 0;
-            return plan_via_dense_copy(p, dry, forced, dense, false, RepackArgs{nullptr, nullptr, n, batch, 1, n, 0, od, 0});
+            return plan_via_dense_copy(p, dry, forced, dense, false, RepackArgs{nullptr, nullptr, n, batch, 1, n, 0, od, 0, (int32_t)rs, 0});
         }
         if (b.rows_fusable(n)) {
             st = b.real_rows(RR_C2R, n, batch, B_IN, 0, id, B_OUT, 0, od / 2, false);
@@ -1237,7 +1234,7 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         if ((ip % 2) || (id % 2)) {
             gfft_desc dense = d;
             dense.in_stride = dense.in_dist = 0;
-            return plan_via_dense_copy(p, dry, forced, dense, true, RepackArgs{nullptr, nullptr, M, N, batch, ip, id, M, N * M});
+            return plan_via_dense_copy(p, dry, forced, dense, true, RepackArgs{nullptr, nullptr, M, N, batch, ip, id, M, N * M, (int32_t)rs, 0});
         }
         const int64_t half = M / 2;
         // row r2c (half spectrum, M/2+1 bins) on every row of every matrix
@@ -1289,7 +1286,7 @@ int build_plan(gfft_plan_s *p, bool dry, const PlanOpts *forced = nullptr)
         if ((op % 2) || (od % 2)) {
             gfft_desc dense = d;
             dense.out_stride = dense.out_dist = 0;
-            return plan_via_dense_copy(p, dry, forced, dense, false, RepackArgs{nullptr, nullptr, M, N, batch, M, N * M, op, od});
+            return plan_via_dense_copy(p, dry, forced, dense, false, RepackArgs{nullptr, nullptr, M, N, batch, M, N * M, op, od, (int32_t)rs, 0});
         }
         const int64_t bsz = N * vcols;  // region B: batch x N x vcols, row pitch vcols
         const size_t c0 = p->launches.size();
@@ -1517,11 +1514,15 @@ int validate(const gfft_plan_s *p)
             const RepackArgs &a = l.rp;
             if (a.width < 1 || a.rows < 1 || a.batch < 1)
                 return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d repack shape", idx);
+            if (a.elem_bytes != 4 && a.elem_bytes != 8)
+                return fail(GFFT_ERR_UNSUPPORTED, "internal: launch %d repack element size", idx);
             if ((st = inside(l.in_buf, l.in_off,
-                             ((a.batch - 1) * a.in_dist + (a.rows - 1) * a.in_pitch + a.width) * rs, "repack load")))
+                             ((a.batch - 1) * a.in_dist + (a.rows - 1) * a.in_pitch + a.width) * a.elem_bytes,
+                             "repack load")))
                 return st;
             if ((st = inside(l.out_buf, l.out_off,
-                             ((a.batch - 1) * a.out_dist + (a.rows - 1) * a.out_pitch + a.width) * rs, "repack store")))
+                             ((a.batch - 1) * a.out_dist + (a.rows - 1) * a.out_pitch + a.width) * a.elem_bytes,
+                             "repack store")))
                 return st;
             break;
         }
@@ -1805,7 +1806,7 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             RepackArgs a = l.rp;
             a.in = ip;
             a.out = op;
-            HIP_TRY(launch_repack(p->prec, a, s));
+            HIP_TRY(launch_repack(a, s));
             break;
         }
         case LT_MIX_RESET:
@@ -2774,7 +2775,8 @@ struct gfft_filter_plan_s {
     int prec = 0, device = 0;
     int64_t n = 0, batch = 0;
     void *tw = nullptr;        // W_n^k, k < n/2
-    const void *pass_tw = nullptr;  // shared device table (not owned)
+    const void *pass_tw = nullptr;      // shared device tables (not owned): the forward FFT's passes,
+    const void *pass_tw_inv = nullptr;  // and the inverse's (its own radix order)
     void *partial = nullptr;   // batch x (n/2 + 1) complex
     StreamOrder order;         // the partial buffer is shared by backward calls
     ~gfft_filter_plan_s()
@@ -2797,9 +2799,11 @@ int gfft_filter_plan_create(gfft_filter_plan *plan, int precision, int64_t n, in
     k.prec = precision;
     k.L = (int)L;
     int nt = 0;
-    if (!is_pow2(n) || n < 64 || (k.nrad = filter_shape(precision, (int)L, &nt, k.rad)) == 0)
+    if (!is_pow2(n) || n < 64 || (k.nrad = filter_shape(precision, (int)L, &nt, k.rad, 0)) == 0)
         return fail(GFFT_ERR_UNSUPPORTED, "fused filter length %lld not supported (64 .. %d)", (long long)n,
                     precision ? 8192 : 16384);
+    KernelEntry ki = k;
+    ki.nrad = filter_shape(precision, (int)L, &nt, ki.rad, 1);
     int dev = 0;
     if (!device_ok(&dev)) return fail(GFFT_ERR_NO_DEVICE, "no HIP device available");
     std::unique_ptr<gfft_filter_plan_s> p(new gfft_filter_plan_s());
@@ -2809,6 +2813,7 @@ int gfft_filter_plan_create(gfft_filter_plan *plan, int precision, int64_t n, in
     p->batch = batch;
     int st = get_pass_twiddles(dev, k, &p->pass_tw, false);
     if (st) return st;
+    if ((st = get_pass_twiddles(dev, ki, &p->pass_tw_inv, false))) return st;
     std::vector<long double> re(L), im(L);
     for (int64_t m = 0; m < L; m++) twiddle_ld(m, n, &re[m], &im[m]);
     if (precision) {
@@ -2843,6 +2848,7 @@ int gfft_filter_forward(gfft_filter_plan plan, const void *x, const void *H, voi
     a.y = y;
     a.tw = plan->tw;
     a.pass_tw = plan->pass_tw;
+    a.pass_tw_inv = plan->pass_tw_inv;
     a.batch = plan->batch;
     a.bins = plan->n / 2 + 1;
     a.conj_h = conj_h ? 1 : 0;
@@ -2863,6 +2869,7 @@ int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, co
     a.partial = plan->partial;
     a.tw = plan->tw;
     a.pass_tw = plan->pass_tw;
+    a.pass_tw_inv = plan->pass_tw_inv;
     a.batch = plan->batch;
     a.bins = plan->n / 2 + 1;
     const hipStream_t s = (hipStream_t)stream;
