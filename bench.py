@@ -98,14 +98,21 @@ def parse():
 
 
 def tile_launches(text):
+    """The launch lines of a plan description, each with its launch index
+    (every line after the `plan` header is one launch, in execution order)."""
     out = []
+    idx = -1
     for ln in text.splitlines():
+        if not ln.strip() or ln.startswith("plan "):
+            continue
+        idx += 1
         if ln.startswith("dual"):
             # two tile passes in one launch: "dual ... | <role A fields> | <role B fields>"
             parts = [p.strip() for p in ln.split("|")]
             f = dict(kv.split("=", 1) for kv in parts[0].split()[1:] if "=" in kv)
             f["kind"] = "dual"
             f["roles"] = [dict(kv.split("=", 1) for kv in p.split() if "=" in kv) for p in parts[1:]]
+            f["index"] = idx
             out.append(f)
             continue
         if not ln.startswith(("tile", "r2c_rows_unpack", "c2r_pack_cols", "real_rows", "unpack", "pack", "hfill",
@@ -113,6 +120,7 @@ def tile_launches(text):
             continue
         f = dict(kv.split("=", 1) for kv in ln.split()[1:] if "=" in kv)
         f["kind"] = ln.split()[0]
+        f["index"] = idx
         out.append(f)
     return out
 
@@ -627,14 +635,11 @@ def main():
         alg_bytes = 2.0 * n * csize * batch
     gflops = flops / (ms_step * 1e-3) / 1e9
 
-    # per-launch profile (HIP events around each kernel on the execution stream)
+    # per-launch kernel time: each launch repeated back to back between one
+    # HIP event pair on the execution stream (an event pair around every
+    # single launch adds its own cost to the kernel's)
     launches = tile_launches(plan.describe())
-    reps = 10
-    acc = [0.0] * len(launches)
-    for _ in range(reps):
-        ms = plan.profile(x, y, inverse=inv)
-        for i, v in enumerate(ms[:len(launches)]):
-            acc[i] += v / reps
+    acc = [plan.profile_launch(x, y, l["index"], reps=20, inverse=inv) for l in launches]
     groups = {}
     for l, t in zip(launches, acc):
         if l["kind"] == "tile":
@@ -702,6 +707,9 @@ def main():
                 pattern["frac_of_pattern"] = round(achieved / max(pats[dom_key].values()), 4)
     del src, dst
 
+    # the launches were repeated one by one above: one whole call restores y
+    plan.execute(x, y, inverse=inv)
+    torch.cuda.synchronize()
     acc_line = accuracy(kind, x, y, n, n2, prec, inv) if not half else None
     if half:  # 16-bit storage: against the float64 FFT of the stored (dequantized) input
         acc_line = accuracy("c2c", x, y, n, n2, prec)

@@ -170,6 +170,19 @@ class Plan:
                                                 ctypes.byref(cnt)))
         return [ms[i] for i in range(min(cnt.value, len(ms)))]
 
+    def profile_launch(self, inp: torch.Tensor, out: torch.Tensor, launch: int, reps: int = 20,
+                       inverse: bool = False, stream=None) -> float:
+        """Mean duration in ms of launch `launch` run `reps` times back to back
+        between one HIP event pair on the execution stream."""
+        dev = self._check(inp, out)
+        ms = ctypes.c_float()
+        with torch.cuda.device(dev):
+            C.check(self._lib.gfft_plan_profile_launch(self._h, inp.data_ptr(), out.data_ptr(),
+                                                       C.GFFT_INVERSE if inverse else C.GFFT_FORWARD,
+                                                       _stream_handle(_dev_idx(dev), stream), int(launch),
+                                                       int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def workspace_bytes(self) -> int:
         v = ctypes.c_size_t()
         C.check(self._lib.gfft_plan_workspace(self._h, ctypes.byref(v)))

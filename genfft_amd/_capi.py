@@ -56,6 +56,7 @@ EXPORTS = (
     "gfft_plan_describe",
     "gfft_plan_explain",
     "gfft_plan_profile",
+    "gfft_plan_profile_launch",
     "gfft_separate_2x_real",
     "gfft_interleave_real",
     "gfft_interleave_real_2d",
@@ -147,6 +148,8 @@ def lib() -> ctypes.CDLL:
         L.gfft_plan_profile.argtypes = [vp, vp, vp, i32, vp, ctypes.POINTER(ctypes.c_float), i32,
                                         ctypes.POINTER(i32)]
         L.gfft_plan_profile.restype = i32
+        L.gfft_plan_profile_launch.argtypes = [vp, vp, vp, i32, vp, i32, i32, ctypes.POINTER(ctypes.c_float)]
+        L.gfft_plan_profile_launch.restype = i32
         L.gfft_separate_2x_real.argtypes = [i32, vp, vp, vp, i64, i64, vp]
         L.gfft_separate_2x_real.restype = i32
         L.gfft_interleave_real.argtypes = [i32, vp, vp, vp, i64, i64, vp]

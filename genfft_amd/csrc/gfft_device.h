@@ -520,8 +520,14 @@ __device__ __forceinline__ void store_tile(const cx<T> *lds, const TileArgs &a, 
                     x0 = swap_ri(x0);
                     x1 = swap_ri(x1);
                 }
-                float4 q = {x0.re, x0.im, x1.re, x1.im};
-                *p = q;
+                // streaming (nontemporal) stores: the tile's output is not
+                // read again by this launch.  Measured on c64 2^20 x 128:
+                // transposing row pass 0.392 -> 0.380 ms, column pass
+                // 0.441 -> 0.435 ms (a nontemporal hint on the loads instead
+                // made the row pass slower)
+                typedef float f4v __attribute__((ext_vector_type(4)));
+                const f4v q = {x0.re, x0.im, x1.re, x1.im};
+                __builtin_nontemporal_store(q, reinterpret_cast<f4v *>(p));
             }
             return;
         }

@@ -17,9 +17,18 @@ struct FilterArgs {
     const void *pass_tw_inv;  // the same for the inverse FFT's radix order
     int64_t batch;
     int64_t bins;         // n/2 + 1
+    int64_t ppitch;       // complex elements between partial rows: bins rounded up to 128 bytes
     int32_t conj_h;       // forward kernel with conj(H) (the x-adjoint alone)
     int32_t pad;
 };
+
+// Partial-row pitch: whole 128-byte lines per row, so the reduction's
+// 32-bin column segments never straddle an extra line.
+inline int64_t filter_partial_pitch(int64_t bins, int prec)
+{
+    const int64_t per_line = prec ? 8 : 16;
+    return (bins + per_line - 1) / per_line * per_line;
+}
 
 struct ReduceArgs {
     const void *src;      // rows x pitch complex partials

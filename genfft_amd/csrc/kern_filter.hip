@@ -351,7 +351,7 @@ struct PairFilter<T, L_, NT_, Radices<RS...>, Radices<RI...>> {
         const cx<T> *gy = reinterpret_cast<const cx<T> *>(a.gy) + row * L;
         const cx<T> *x = reinterpret_cast<const cx<T> *>(a.x) + row * L;
         cx<T> *gx = reinterpret_cast<cx<T> *>(a.y) + row * L;
-        cx<T> *part = reinterpret_cast<cx<T> *>(a.partial) + row * (L + 1);
+        cx<T> *part = reinterpret_cast<cx<T> *>(a.partial) + row * a.ppitch;
         const cx<T> *H = reinterpret_cast<const cx<T> *>(a.H);
         const cx<T> *tw = reinterpret_cast<const cx<T> *>(a.tw);
         const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(NT) filter_bwd_kernel(FilterArgs a)
     const cx<T> *gy = reinterpret_cast<const cx<T> *>(a.gy) + row * L;
     const cx<T> *x = reinterpret_cast<const cx<T> *>(a.x) + row * L;
     cx<T> *gx = reinterpret_cast<cx<T> *>(a.y) + row * L;
-    cx<T> *part = reinterpret_cast<cx<T> *>(a.partial) + row * (L + 1);
+    cx<T> *part = reinterpret_cast<cx<T> *>(a.partial) + row * a.ppitch;
     const cx<T> *H = reinterpret_cast<const cx<T> *>(a.H);
     const cx<T> *tw = reinterpret_cast<const cx<T> *>(a.tw);
     const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
@@ -621,15 +621,15 @@ hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s
         r.src = a.partial;
         r.bins = bins;
         r.rows = rows;
-        r.pitch = bins;
+        r.pitch = a.ppitch;
         r.seg = seg;
         if (S > 1) {
             r.dst = a.partial;  // segment sums into the first row of each segment
-            r.out_pitch = seg * bins;
+            r.out_pitch = seg * a.ppitch;
             if (prec) hipLaunchKernelGGL(filter_reduce_kernel<double>, dim3((unsigned)gx, (unsigned)S), dim3(32 * kReduceLanes), 0, s, r);
             else hipLaunchKernelGGL(filter_reduce_kernel<float>, dim3((unsigned)gx, (unsigned)S), dim3(32 * kReduceLanes), 0, s, r);
             r.rows = S;
-            r.pitch = seg * bins;
+            r.pitch = seg * a.ppitch;
             r.seg = S;
         }
         r.dst = a.y;

@@ -1889,8 +1889,14 @@ int check_noalloc_fit(const gfft_plan_s *p)
 //   longer lengths            per length class, the single-pass or four-step
 //                              tile launches with job tables along the
 //                              transform index.
-// The whole list is captured once into a HIP graph and replayed.
+// The classes are dealt to at most kBatchBranches branches, balanced by bytes
+// (the mixed launch on branch 0); each branch has its own scratch region, so
+// the branches are independent.  Eager execution runs them one after the
+// other on the caller's stream; the graph is captured with every branch but
+// the first on a side stream forked from the capture stream and joined back,
+// so its replay may run them concurrently.
 // ---------------------------------------------------------------------------
+constexpr int kBatchBranches = 4;  // the hardware queues a process gets by default
 }  // namespace
 }  // namespace gfft
 
@@ -1913,6 +1919,9 @@ struct gfft_batch_plan_s {
     const void *g_in = nullptr;
     void *g_out = nullptr;
     hipStream_t cap_stream = nullptr;
+    std::vector<std::vector<gfft::Launch>> branches;  // launches split by branch (graph capture)
+    hipStream_t side[gfft::kBatchBranches - 1] = {};   // capture streams of branches 1 ..
+    hipEvent_t fork = nullptr, join[gfft::kBatchBranches - 1] = {};
     std::map<int64_t, gfft_plan> per_job;  // one single-transform plan per length (reference loop)
     StreamOrder order;  // scratch, counter and the captured graph are shared by all calls
 };
@@ -1995,13 +2004,51 @@ int build_batch(gfft_batch_plan_s *p)
         p->launches.push_back(l);
     }
 
-    // length classes above the mixed kernel: job tables along the transform index
+    // length classes above the mixed kernel: job tables along the transform
+    // index.  Branches first: the classes, largest first, each to the branch
+    // with the fewest bytes so far (the mixed launch's jobs count for branch 0).
+    int64_t mix_elems = 0;
+    for (const MixTask &t : tasks) mix_elems += (int64_t)t.count << t.lg;
+    const int nbr = (int)std::min<int64_t>(kBatchBranches, (tasks.empty() ? 0 : 1) + (int64_t)big.size());
+    std::vector<int64_t> load(std::max(nbr, 1), 0);
+    load[0] = mix_elems;
+    std::vector<int> cls_branch(big.size(), 0);
+    {
+        std::vector<size_t> by_size(big.size());
+        for (size_t i = 0; i < big.size(); i++) by_size[i] = i;
+        std::stable_sort(by_size.begin(), by_size.end(), [&](size_t x, size_t y) {
+            return big[x].count << big[x].lg > big[y].count << big[y].lg;
+        });
+        for (size_t i : by_size) {
+            int bb = 0;
+            for (int q = 1; q < nbr; q++)
+                if (load[q] < load[bb]) bb = q;
+            cls_branch[i] = bb;
+            load[bb] += big[i].count << big[i].lg;
+        }
+    }
+    // one scratch region per branch: the largest four-step chunk among its classes
+    auto four_step_chunk = [&](const Cls &c) {
+        const int64_t L = (int64_t)1 << c.lg;
+        return std::max<int64_t>(1, std::min<int64_t>(c.count, ((int64_t)1 << 30) / (L * (int64_t)cs)));
+    };
+    const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
+    std::vector<int64_t> region(std::max(nbr, 1), 0), region_off(std::max(nbr, 1), 0);
+    for (size_t i = 0; i < big.size(); i++) {
+        const int64_t L = (int64_t)1 << big[i].lg;
+        if (L > maxrow) region[cls_branch[i]] = std::max(region[cls_branch[i]], four_step_chunk(big[i]) * L);
+    }
+    int64_t scratch_elems = 0;
+    for (int q = 0; q < nbr; q++) {
+        region_off[q] = scratch_elems;
+        scratch_elems += region[q];
+    }
+
     gfft_plan_s tmp;
     tmp.prec = prec;
     tmp.device = p->device;
     tmp.num_cus = p->num_cus;
     Builder b{&tmp, prec, cs, false};
-    const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
     auto set_jobs = [&](int64_t first, bool in_side, bool out_side) {
         TileArgs &a = tmp.launches.back().ta;
         if (in_side) {
@@ -2020,7 +2067,11 @@ int build_batch(gfft_batch_plan_s *p)
     };
     int cnt = 0;
     const KernelEntry *tab = tile_table(prec, 0, &cnt);
-    for (const Cls &c : big) {
+    p->branches.assign(std::max(nbr, 1), {});
+    for (const Launch &l : p->launches) p->branches[0].push_back(l);  // counter reset, mixed launch
+    for (size_t ci = 0; ci < big.size(); ci++) {
+        const Cls &c = big[ci];
+        const size_t first_launch = tmp.launches.size();
         const int64_t L = (int64_t)1 << c.lg;
         // each length class takes the kernel choices a single plan of that
         // length would: the tuning table's entry, or the defaults
@@ -2034,34 +2085,80 @@ int build_batch(gfft_batch_plan_s *p)
                              TW_NONE, 0, false, false, variant)))
                 return st;
             set_jobs(c.first, true, true);
-            continue;
+        } else {
+            int l1, l2;
+            four_step_split(prec, c.lg, &l1, &l2, b.opts.n1_log2);
+            const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
+            const int64_t chunk = four_step_chunk(c);
+            const int64_t so = region_off[cls_branch[ci]];  // this branch's scratch region
+            for (int64_t b0 = 0; b0 < c.count; b0 += chunk) {
+                const int64_t cb = std::min(chunk, c.count - b0);
+                if ((st = b.tile((int)N1, PAT_COL, PAT_COL, b.opts.col_m, B_IN, 0, B_SCR, so, N2, cb, 1, 0, 0, 1, N2, 0,
+                                 L, 1, N2, TW_INNER, L, false, false)))
+                    return st;
+                set_jobs(c.first + b0, true, false);
+                if ((st = b.tile((int)N2, PAT_ROW, PAT_COL, b.opts.row_m, B_SCR, so, B_OUT, 0, N1, cb, 1, 0, L, N2, 1,
+                                 0, 0, 1, N1, TW_NONE, 0, false, false)))
+                    return st;
+                set_jobs(c.first + b0, false, true);
+            }
         }
-        int l1, l2;
-        four_step_split(prec, c.lg, &l1, &l2, b.opts.n1_log2);
-        const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
-        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(c.count, ((int64_t)1 << 30) / (L * (int64_t)cs)));
-        b.need_scratch(chunk * L);
-        for (int64_t b0 = 0; b0 < c.count; b0 += chunk) {
-            const int64_t cb = std::min(chunk, c.count - b0);
-            if ((st = b.tile((int)N1, PAT_COL, PAT_COL, b.opts.col_m, B_IN, 0, B_SCR, 0, N2, cb, 1, 0, 0, 1, N2, 0, L,
-                             1, N2, TW_INNER, L, false, false)))
-                return st;
-            set_jobs(c.first + b0, true, false);
-            if ((st = b.tile((int)N2, PAT_ROW, PAT_COL, b.opts.row_m, B_SCR, 0, B_OUT, 0, N1, cb, 1, 0, L, N2, 1, 0, 0,
-                             1, N1, TW_NONE, 0, false, false)))
-                return st;
-            set_jobs(c.first + b0, false, true);
+        for (size_t i = first_launch; i < tmp.launches.size(); i++) p->branches[cls_branch[ci]].push_back(tmp.launches[i]);
+    }
+    // bounds proof of the scratch side: every launch of branch q stays inside
+    // region q, so branches running at the same time never share bytes
+    for (int q = 0; q < (int)p->branches.size(); q++) {
+        const int64_t lo_b = region_off[q] * (int64_t)cs, hi_b = (region_off[q] + region[q]) * (int64_t)cs;
+        for (const Launch &l : p->branches[q]) {
+            if (l.type != LT_TILE) continue;
+            const TileArgs &a = l.ta;
+            const int64_t n_outer2 = a.n_tiles / ((int64_t)a.tiles_inner * a.n_outer1);
+            for (int side = 0; side < 2; side++) {
+                if ((side ? l.out_buf : l.in_buf) != B_SCR) continue;
+                const int64_t last = side ? (n_outer2 - 1) * a.out_outer2 + (int64_t)(a.n_outer1 - 1) * a.out_outer +
+                                                (int64_t)(a.n_inner - 1) * a.out_inner + (int64_t)(l.k->L - 1) * a.out_elem
+                                          : (n_outer2 - 1) * a.in_outer2 + (int64_t)(a.n_outer1 - 1) * a.in_outer +
+                                                (int64_t)(a.n_inner - 1) * a.in_inner + (int64_t)(l.k->L - 1) * a.in_elem;
+                const int64_t off = side ? l.out_off : l.in_off;
+                if (off < lo_b || off + (last + 1) * (int64_t)cs > hi_b)
+                    return fail(GFFT_ERR_UNSUPPORTED, "internal: batch branch %d leaves its scratch region", q);
+            }
         }
     }
-    for (auto &l : tmp.launches) p->launches.push_back(l);
-    p->scratch_bytes = (size_t)b.scratch_elems * cs;
+    // eager order: the branches one after the other
+    p->launches.clear();
+    for (const auto &br : p->branches)
+        for (const Launch &l : br) p->launches.push_back(l);
+    p->scratch_bytes = (size_t)scratch_elems * cs;
     if (p->scratch_bytes) HIP_TRY(hipMalloc(&p->scratch, p->scratch_bytes));
     return GFFT_OK;
 }
 
-int enqueue_batch(gfft_batch_plan_s *p, const void *in, void *out, hipStream_t s)
+// Eager: every launch on s.  Forked (graph capture): branch 0 on s, every
+// other branch on its side stream, which waits for an event recorded on s
+// first and is joined back into s at the end.
+int enqueue_batch(gfft_batch_plan_s *p, const void *in, void *out, hipStream_t s, bool forked = false)
 {
-    return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, s);
+    const int nbr = (int)p->branches.size();
+    if (!forked || nbr <= 1)
+        return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, s);
+    if (!p->fork) HIP_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+    for (int q = 1; q < nbr; q++) {
+        if (!p->side[q - 1]) HIP_TRY(hipStreamCreateWithFlags(&p->side[q - 1], hipStreamNonBlocking));
+        if (!p->join[q - 1]) HIP_TRY(hipEventCreateWithFlags(&p->join[q - 1], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(p->fork, s));
+    for (int q = 1; q < nbr; q++) {
+        hipStream_t t = p->side[q - 1];
+        HIP_TRY(hipStreamWaitEvent(t, p->fork, 0));
+        int st = run_launches(p->branches[q], p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, t);
+        if (st) return st;
+        HIP_TRY(hipEventRecord(p->join[q - 1], t));
+    }
+    int st = run_launches(p->branches[0], p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, s);
+    if (st) return st;
+    for (int q = 1; q < nbr; q++) HIP_TRY(hipStreamWaitEvent(s, p->join[q - 1], 0));
+    return GFFT_OK;
 }
 
 
@@ -2584,6 +2681,50 @@ int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int direction, 
     return st;
 }
 
+int gfft_plan_profile_launch(gfft_plan plan, const void *in, void *out, int direction, void *stream, int launch,
+                             int reps, float *ms)
+{
+    if (!plan || !in || !out || !ms) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    if (launch < 0 || launch >= (int)plan->launches.size())
+        return fail(GFFT_ERR_INVALID_ARG, "launch %d out of range (plan has %zu)", launch, plan->launches.size());
+    if (reps < 1 || reps > 10000) return fail(GFFT_ERR_INVALID_ARG, "reps %d out of range 1 .. 10000", reps);
+    const Launch &l = plan->launches[launch];
+    if (l.type == LT_MIX || l.type == LT_MIX_RESET)
+        return fail(GFFT_ERR_UNSUPPORTED, "a work-queue launch runs once per reset");
+    if (plan->scratch_bytes && !plan->scratch)
+        return fail(GFFT_ERR_INVALID_ARG, "plan created with GFFT_FLAG_NO_WORKSPACE holds no workspace");
+    if (int st = on_plan_device(plan->device)) return st;
+    const hipStream_t s = (hipStream_t)stream;
+    const std::vector<Launch> rep((size_t)reps, l);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    int st = GFFT_OK;
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) st = fail(GFFT_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+    if (!st) {
+        std::lock_guard<std::mutex> lk(plan->order.mu);  // the workspace is shared with other calls
+        e = plan->order.before(s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        if (e != hipSuccess) st = fail(GFFT_ERR_HIP, "%s", hipGetErrorString(e));
+        if (!st) st = run_launches(rep, plan->prec, plan->xcd_remap, in, out, plan->scratch, direction, s);
+        if (!st) {
+            e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = plan->order.after(s);
+            if (e != hipSuccess) st = fail(GFFT_ERR_HIP, "%s", hipGetErrorString(e));
+        }
+    }
+    if (!st) {
+        float t = 0.f;
+        e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) st = fail(GFFT_ERR_HIP, "%s", hipGetErrorString(e));
+        else *ms = t / (float)reps;
+    }
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return st;
+}
+
 int gfft_plan_workspace(gfft_plan plan, size_t *bytes)
 {
     if (!plan || !bytes) return fail(GFFT_ERR_INVALID_ARG, "null argument");
@@ -2683,6 +2824,11 @@ int gfft_batch_plan_destroy(gfft_batch_plan plan)
     if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
     if (plan->graph) (void)hipGraphDestroy(plan->graph);
     if (plan->cap_stream) (void)hipStreamDestroy(plan->cap_stream);
+    for (hipStream_t t : plan->side)
+        if (t) (void)hipStreamDestroy(t);
+    for (hipEvent_t e : plan->join)
+        if (e) (void)hipEventDestroy(e);
+    if (plan->fork) (void)hipEventDestroy(plan->fork);
     for (auto &kv : plan->per_job) gfft_plan_destroy(kv.second);
     for (void *ptr : {(void *)plan->d_job_in, (void *)plan->d_job_out, (void *)plan->d_job_inv,
                       (void *)plan->d_tasks, (void *)plan->d_counter, plan->scratch})
@@ -2738,7 +2884,7 @@ int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode
         }
         if (!plan->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking));
         HIP_TRY(hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_batch(plan, in, out, plan->cap_stream);
+        int st = enqueue_batch(plan, in, out, plan->cap_stream, true);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(plan->cap_stream, &g);
         if (st) {
@@ -2761,8 +2907,8 @@ int gfft_batch_plan_describe(gfft_batch_plan plan, char *buf, size_t len)
 {
     if (!plan || !buf || !len) return fail(GFFT_ERR_INVALID_ARG, "null argument");
     char head[256];
-    snprintf(head, sizeof(head), "batch jobs=%lld scratch_bytes=%zu offsets_even=%d\n", (long long)plan->count,
-             plan->scratch_bytes, (int)plan->offsets_even);
+    snprintf(head, sizeof(head), "batch jobs=%lld scratch_bytes=%zu offsets_even=%d branches=%d\n",
+             (long long)plan->count, plan->scratch_bytes, (int)plan->offsets_even, (int)plan->branches.size());
     return describe_launches(plan->launches, head, buf, len);
 }
 
@@ -2777,7 +2923,7 @@ struct gfft_filter_plan_s {
     void *tw = nullptr;        // W_n^k, k < n/2
     const void *pass_tw = nullptr;      // shared device tables (not owned): the forward FFT's passes,
     const void *pass_tw_inv = nullptr;  // and the inverse's (its own radix order)
-    void *partial = nullptr;   // batch x (n/2 + 1) complex
+    void *partial = nullptr;   // batch rows of n/2 + 1 complex, filter_partial_pitch apart
     StreamOrder order;         // the partial buffer is shared by backward calls
     ~gfft_filter_plan_s()
     {
@@ -2827,7 +2973,7 @@ int gfft_filter_plan_create(gfft_filter_plan *plan, int precision, int64_t n, in
     }
     if (st) return st;
     const size_t cs = precision ? 16 : 8;
-    HIP_TRY(hipMalloc(&p->partial, (size_t)batch * (L + 1) * cs));
+    HIP_TRY(hipMalloc(&p->partial, (size_t)batch * filter_partial_pitch(L + 1, precision) * cs));
     *plan = p.release();
     return GFFT_OK;
 }
@@ -2872,6 +3018,7 @@ int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, co
     a.pass_tw_inv = plan->pass_tw_inv;
     a.batch = plan->batch;
     a.bins = plan->n / 2 + 1;
+    a.ppitch = filter_partial_pitch(a.bins, plan->prec);
     const hipStream_t s = (hipStream_t)stream;
     std::lock_guard<std::mutex> lk(plan->order.mu);
     HIP_TRY(plan->order.before(s));

@@ -203,6 +203,16 @@ GFFT_API int gfft_execute_host(gfft_plan plan, const void *in, void *out, int di
 GFFT_API int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int direction, void *stream,
                                float *ms, int max_launches, int *count);
 
+/* Runs launch `launch` of the plan `reps` times back to back on `stream`
+ * between one HIP event pair, waits, and writes the mean duration of one
+ * launch (milliseconds) to *ms: a kernel's time without the cost of an event
+ * pair around each launch.  The buffers are those of a gfft_execute call
+ * (no staging copy is made); repeating a launch of an in-place plan changes
+ * the data, not the timing.  GFFT_ERR_UNSUPPORTED for the launches of an
+ * uneven batch's work queue.  No reference counterpart. */
+GFFT_API int gfft_plan_profile_launch(gfft_plan plan, const void *in, void *out, int direction, void *stream,
+                                      int launch, int reps, float *ms);
+
 /* Device workspace the plan holds, in bytes. */
 GFFT_API int gfft_plan_workspace(gfft_plan plan, size_t *bytes);
 
@@ -247,7 +257,10 @@ GFFT_API int gfft_interleave_real_2d(int precision, void *out, int64_t out_strid
  * in + in_offsets[j] and writing out + out_offsets[j] (complex elements),
  * forward or (unscaled) inverse per directions[j].  Lengths up to 4096 (c64)
  * / 2048 (c128) run in one persistent launch that drains a device-side task
- * queue; longer lengths run as per-length batched launches. */
+ * queue; longer lengths run as per-length batched launches.  Jobs must not
+ * write where another job reads or writes (a job may be in place): in graph
+ * mode the length classes run concurrently, on up to four branches of the
+ * graph. */
 typedef struct gfft_batch_plan_s *gfft_batch_plan;
 
 #define GFFT_BATCH_GRAPH 0    /* capture the launches once into a HIP graph and replay it */

@@ -294,3 +294,30 @@ def test_overlapped_four_step_matches_sequential_passes(gpu, prec, batch, monkey
     plain.execute(x, want)
     torch.cuda.synchronize()
     assert torch.equal(z, want)
+
+
+def test_profile_launch_times_one_launch(gpu):
+    """gfft_plan_profile_launch: each launch of a four-step plan repeated
+    between one event pair gives a positive time, leaves an out-of-place
+    result unchanged (the passes read the input and the scratch only), and
+    rejects a launch index outside the plan and non-positive reps."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    n, batch = 1 << 20, 4
+    g = torch.Generator(device=gpu).manual_seed(5)
+    x = torch.randn(batch, n, dtype=torch.complex64, device=gpu, generator=g)
+    p = G.Plan(C.GFFT_C2C, n, precision=0, batch=batch)
+    y = torch.empty_like(x)
+    p.execute(x, y)
+    torch.cuda.synchronize()
+    want = y.clone()
+    nl = len([ln for ln in p.describe().splitlines() if ln.strip() and not ln.startswith("plan ")])
+    assert nl == 2
+    for i in range(nl):
+        ms = p.profile_launch(x, y, i, reps=5)
+        assert 0.0 < ms < 1000.0
+    torch.cuda.synchronize()
+    assert torch.equal(y, want)
+    for bad_launch, bad_reps in ((nl, 1), (-1, 1), (0, 0)):
+        with pytest.raises(G.GfftError):
+            p.profile_launch(x, y, bad_launch, reps=bad_reps)
