@@ -520,11 +520,12 @@ __device__ __forceinline__ void store_tile(const cx<T> *lds, const TileArgs &a, 
                     x0 = swap_ri(x0);
                     x1 = swap_ri(x1);
                 }
-                // streaming (nontemporal) stores: the tile's output is not
-                // read again by this launch.  Measured on c64 2^20 x 128:
-                // transposing row pass 0.392 -> 0.380 ms, column pass
-                // 0.441 -> 0.435 ms (a nontemporal hint on the loads instead
-                // made the row pass slower)
+                // streaming (nontemporal) stores: nothing in this launch
+                // reads the tile's output again.  Measured on c64 2^20 x 128:
+                // transposing row pass 0.392 -> 0.380 ms and column pass
+                // 0.441 -> 0.435 ms on one MI355X, no change on another.  (A
+                // runtime choice between this and a plain store is merged
+                // into one plain store by the compiler, which drops the hint.)
                 typedef float f4v __attribute__((ext_vector_type(4)));
                 const f4v q = {x0.re, x0.im, x1.re, x1.im};
                 __builtin_nontemporal_store(q, reinterpret_cast<f4v *>(p));
@@ -664,6 +665,9 @@ __device__ __forceinline__ void store_tile_h(const cx<float> *lds, const TileArg
     };
     if constexpr (TOT % (4 * NT) == 0 && (OUTP == PAT_ROW ? L : M) % 4 == 0) {
         if (a.vec_out && nvalid == M) {
+            // plain stores: with the streaming hint each 16-bit pass ran faster
+            // alone, the plan did not (FP16 2^20 x 64: the row pass then found
+            // the 256 MiB intermediate outside the Infinity Cache)
 #pragma unroll
             for (int it = 0; it < TOT / (4 * NT); it++) {
                 const int e = (tid + it * NT) * 4;

@@ -1889,14 +1889,11 @@ int check_noalloc_fit(const gfft_plan_s *p)
 //   longer lengths            per length class, the single-pass or four-step
 //                              tile launches with job tables along the
 //                              transform index.
-// The classes are dealt to at most kBatchBranches branches, balanced by bytes
-// (the mixed launch on branch 0); each branch has its own scratch region, so
-// the branches are independent.  Eager execution runs them one after the
-// other on the caller's stream; the graph is captured with every branch but
-// the first on a side stream forked from the capture stream and joined back,
-// so its replay may run them concurrently.
+// The whole list is captured once into a HIP graph and replayed.  (Captured
+// with the classes forked onto four side streams, the replay ran slower than
+// one stream: the persistent class kernels each size their grid to the whole
+// GPU and contend when they overlap.)
 // ---------------------------------------------------------------------------
-constexpr int kBatchBranches = 4;  // the hardware queues a process gets by default
 }  // namespace
 }  // namespace gfft
 
@@ -1919,9 +1916,6 @@ struct gfft_batch_plan_s {
     const void *g_in = nullptr;
     void *g_out = nullptr;
     hipStream_t cap_stream = nullptr;
-    std::vector<std::vector<gfft::Launch>> branches;  // launches split by branch (graph capture)
-    hipStream_t side[gfft::kBatchBranches - 1] = {};   // capture streams of branches 1 ..
-    hipEvent_t fork = nullptr, join[gfft::kBatchBranches - 1] = {};
     std::map<int64_t, gfft_plan> per_job;  // one single-transform plan per length (reference loop)
     StreamOrder order;  // scratch, counter and the captured graph are shared by all calls
 };
@@ -2005,50 +1999,14 @@ int build_batch(gfft_batch_plan_s *p)
     }
 
     // length classes above the mixed kernel: job tables along the transform
-    // index.  Branches first: the classes, largest first, each to the branch
-    // with the fewest bytes so far (the mixed launch's jobs count for branch 0).
-    int64_t mix_elems = 0;
-    for (const MixTask &t : tasks) mix_elems += (int64_t)t.count << t.lg;
-    const int nbr = (int)std::min<int64_t>(kBatchBranches, (tasks.empty() ? 0 : 1) + (int64_t)big.size());
-    std::vector<int64_t> load(std::max(nbr, 1), 0);
-    load[0] = mix_elems;
-    std::vector<int> cls_branch(big.size(), 0);
-    {
-        std::vector<size_t> by_size(big.size());
-        for (size_t i = 0; i < big.size(); i++) by_size[i] = i;
-        std::stable_sort(by_size.begin(), by_size.end(), [&](size_t x, size_t y) {
-            return big[x].count << big[x].lg > big[y].count << big[y].lg;
-        });
-        for (size_t i : by_size) {
-            int bb = 0;
-            for (int q = 1; q < nbr; q++)
-                if (load[q] < load[bb]) bb = q;
-            cls_branch[i] = bb;
-            load[bb] += big[i].count << big[i].lg;
-        }
-    }
-    // one scratch region per branch: the largest four-step chunk among its classes
-    auto four_step_chunk = [&](const Cls &c) {
-        const int64_t L = (int64_t)1 << c.lg;
-        return std::max<int64_t>(1, std::min<int64_t>(c.count, ((int64_t)1 << 30) / (L * (int64_t)cs)));
-    };
-    const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
-    std::vector<int64_t> region(std::max(nbr, 1), 0), region_off(std::max(nbr, 1), 0);
-    for (size_t i = 0; i < big.size(); i++) {
-        const int64_t L = (int64_t)1 << big[i].lg;
-        if (L > maxrow) region[cls_branch[i]] = std::max(region[cls_branch[i]], four_step_chunk(big[i]) * L);
-    }
-    int64_t scratch_elems = 0;
-    for (int q = 0; q < nbr; q++) {
-        region_off[q] = scratch_elems;
-        scratch_elems += region[q];
-    }
-
+    // index, one class after another, the four-step classes sharing one
+    // scratch region
     gfft_plan_s tmp;
     tmp.prec = prec;
     tmp.device = p->device;
     tmp.num_cus = p->num_cus;
     Builder b{&tmp, prec, cs, false};
+    const int maxrow = max_len(prec, PAT_ROW, PAT_ROW);
     auto set_jobs = [&](int64_t first, bool in_side, bool out_side) {
         TileArgs &a = tmp.launches.back().ta;
         if (in_side) {
@@ -2067,11 +2025,7 @@ int build_batch(gfft_batch_plan_s *p)
     };
     int cnt = 0;
     const KernelEntry *tab = tile_table(prec, 0, &cnt);
-    p->branches.assign(std::max(nbr, 1), {});
-    for (const Launch &l : p->launches) p->branches[0].push_back(l);  // counter reset, mixed launch
-    for (size_t ci = 0; ci < big.size(); ci++) {
-        const Cls &c = big[ci];
-        const size_t first_launch = tmp.launches.size();
+    for (const Cls &c : big) {
         const int64_t L = (int64_t)1 << c.lg;
         // each length class takes the kernel choices a single plan of that
         // length would: the tuning table's entry, or the defaults
@@ -2085,80 +2039,50 @@ int build_batch(gfft_batch_plan_s *p)
                              TW_NONE, 0, false, false, variant)))
                 return st;
             set_jobs(c.first, true, true);
-        } else {
-            int l1, l2;
-            four_step_split(prec, c.lg, &l1, &l2, b.opts.n1_log2);
-            const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
-            const int64_t chunk = four_step_chunk(c);
-            const int64_t so = region_off[cls_branch[ci]];  // this branch's scratch region
-            for (int64_t b0 = 0; b0 < c.count; b0 += chunk) {
-                const int64_t cb = std::min(chunk, c.count - b0);
-                if ((st = b.tile((int)N1, PAT_COL, PAT_COL, b.opts.col_m, B_IN, 0, B_SCR, so, N2, cb, 1, 0, 0, 1, N2, 0,
-                                 L, 1, N2, TW_INNER, L, false, false)))
-                    return st;
-                set_jobs(c.first + b0, true, false);
-                if ((st = b.tile((int)N2, PAT_ROW, PAT_COL, b.opts.row_m, B_SCR, so, B_OUT, 0, N1, cb, 1, 0, L, N2, 1,
-                                 0, 0, 1, N1, TW_NONE, 0, false, false)))
-                    return st;
-                set_jobs(c.first + b0, false, true);
-            }
+            continue;
         }
-        for (size_t i = first_launch; i < tmp.launches.size(); i++) p->branches[cls_branch[ci]].push_back(tmp.launches[i]);
-    }
-    // bounds proof of the scratch side: every launch of branch q stays inside
-    // region q, so branches running at the same time never share bytes
-    for (int q = 0; q < (int)p->branches.size(); q++) {
-        const int64_t lo_b = region_off[q] * (int64_t)cs, hi_b = (region_off[q] + region[q]) * (int64_t)cs;
-        for (const Launch &l : p->branches[q]) {
-            if (l.type != LT_TILE) continue;
-            const TileArgs &a = l.ta;
-            const int64_t n_outer2 = a.n_tiles / ((int64_t)a.tiles_inner * a.n_outer1);
-            for (int side = 0; side < 2; side++) {
-                if ((side ? l.out_buf : l.in_buf) != B_SCR) continue;
-                const int64_t last = side ? (n_outer2 - 1) * a.out_outer2 + (int64_t)(a.n_outer1 - 1) * a.out_outer +
-                                                (int64_t)(a.n_inner - 1) * a.out_inner + (int64_t)(l.k->L - 1) * a.out_elem
-                                          : (n_outer2 - 1) * a.in_outer2 + (int64_t)(a.n_outer1 - 1) * a.in_outer +
-                                                (int64_t)(a.n_inner - 1) * a.in_inner + (int64_t)(l.k->L - 1) * a.in_elem;
-                const int64_t off = side ? l.out_off : l.in_off;
-                if (off < lo_b || off + (last + 1) * (int64_t)cs > hi_b)
-                    return fail(GFFT_ERR_UNSUPPORTED, "internal: batch branch %d leaves its scratch region", q);
-            }
+        int l1, l2;
+        four_step_split(prec, c.lg, &l1, &l2, b.opts.n1_log2);
+        const int64_t N1 = (int64_t)1 << l1, N2 = (int64_t)1 << l2;
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(c.count, ((int64_t)1 << 30) / (L * (int64_t)cs)));
+        b.need_scratch(chunk * L);
+        for (int64_t b0 = 0; b0 < c.count; b0 += chunk) {
+            const int64_t cb = std::min(chunk, c.count - b0);
+            if ((st = b.tile((int)N1, PAT_COL, PAT_COL, b.opts.col_m, B_IN, 0, B_SCR, 0, N2, cb, 1, 0, 0, 1, N2, 0, L,
+                             1, N2, TW_INNER, L, false, false)))
+                return st;
+            set_jobs(c.first + b0, true, false);
+            if ((st = b.tile((int)N2, PAT_ROW, PAT_COL, b.opts.row_m, B_SCR, 0, B_OUT, 0, N1, cb, 1, 0, L, N2, 1, 0, 0,
+                             1, N1, TW_NONE, 0, false, false)))
+                return st;
+            set_jobs(c.first + b0, false, true);
         }
     }
-    // eager order: the branches one after the other
-    p->launches.clear();
-    for (const auto &br : p->branches)
-        for (const Launch &l : br) p->launches.push_back(l);
-    p->scratch_bytes = (size_t)scratch_elems * cs;
+    // bounds proof of the scratch side (the job tables place the other side)
+    const int64_t scratch_bytes = b.scratch_elems * (int64_t)cs;
+    for (const Launch &l : tmp.launches) {
+        const TileArgs &a = l.ta;
+        const int64_t n_outer2 = a.n_tiles / ((int64_t)a.tiles_inner * a.n_outer1);
+        for (int side = 0; side < 2; side++) {
+            if ((side ? l.out_buf : l.in_buf) != B_SCR) continue;
+            const int64_t last = side ? (n_outer2 - 1) * a.out_outer2 + (int64_t)(a.n_outer1 - 1) * a.out_outer +
+                                            (int64_t)(a.n_inner - 1) * a.out_inner + (int64_t)(l.k->L - 1) * a.out_elem
+                                      : (n_outer2 - 1) * a.in_outer2 + (int64_t)(a.n_outer1 - 1) * a.in_outer +
+                                            (int64_t)(a.n_inner - 1) * a.in_inner + (int64_t)(l.k->L - 1) * a.in_elem;
+            const int64_t off = side ? l.out_off : l.in_off;
+            if (off < 0 || off + (last + 1) * (int64_t)cs > scratch_bytes)
+                return fail(GFFT_ERR_UNSUPPORTED, "internal: batch launch leaves the scratch buffer");
+        }
+    }
+    for (auto &l : tmp.launches) p->launches.push_back(l);
+    p->scratch_bytes = (size_t)scratch_bytes;
     if (p->scratch_bytes) HIP_TRY(hipMalloc(&p->scratch, p->scratch_bytes));
     return GFFT_OK;
 }
 
-// Eager: every launch on s.  Forked (graph capture): branch 0 on s, every
-// other branch on its side stream, which waits for an event recorded on s
-// first and is joined back into s at the end.
-int enqueue_batch(gfft_batch_plan_s *p, const void *in, void *out, hipStream_t s, bool forked = false)
+int enqueue_batch(gfft_batch_plan_s *p, const void *in, void *out, hipStream_t s)
 {
-    const int nbr = (int)p->branches.size();
-    if (!forked || nbr <= 1)
-        return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, s);
-    if (!p->fork) HIP_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
-    for (int q = 1; q < nbr; q++) {
-        if (!p->side[q - 1]) HIP_TRY(hipStreamCreateWithFlags(&p->side[q - 1], hipStreamNonBlocking));
-        if (!p->join[q - 1]) HIP_TRY(hipEventCreateWithFlags(&p->join[q - 1], hipEventDisableTiming));
-    }
-    HIP_TRY(hipEventRecord(p->fork, s));
-    for (int q = 1; q < nbr; q++) {
-        hipStream_t t = p->side[q - 1];
-        HIP_TRY(hipStreamWaitEvent(t, p->fork, 0));
-        int st = run_launches(p->branches[q], p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, t);
-        if (st) return st;
-        HIP_TRY(hipEventRecord(p->join[q - 1], t));
-    }
-    int st = run_launches(p->branches[0], p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, s);
-    if (st) return st;
-    for (int q = 1; q < nbr; q++) HIP_TRY(hipStreamWaitEvent(s, p->join[q - 1], 0));
-    return GFFT_OK;
+    return run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, GFFT_FORWARD, s);
 }
 
 
@@ -2824,11 +2748,6 @@ int gfft_batch_plan_destroy(gfft_batch_plan plan)
     if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
     if (plan->graph) (void)hipGraphDestroy(plan->graph);
     if (plan->cap_stream) (void)hipStreamDestroy(plan->cap_stream);
-    for (hipStream_t t : plan->side)
-        if (t) (void)hipStreamDestroy(t);
-    for (hipEvent_t e : plan->join)
-        if (e) (void)hipEventDestroy(e);
-    if (plan->fork) (void)hipEventDestroy(plan->fork);
     for (auto &kv : plan->per_job) gfft_plan_destroy(kv.second);
     for (void *ptr : {(void *)plan->d_job_in, (void *)plan->d_job_out, (void *)plan->d_job_inv,
                       (void *)plan->d_tasks, (void *)plan->d_counter, plan->scratch})
@@ -2884,7 +2803,7 @@ int gfft_batch_execute(gfft_batch_plan plan, const void *in, void *out, int mode
         }
         if (!plan->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&plan->cap_stream, hipStreamNonBlocking));
         HIP_TRY(hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_batch(plan, in, out, plan->cap_stream, true);
+        int st = enqueue_batch(plan, in, out, plan->cap_stream);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(plan->cap_stream, &g);
         if (st) {
@@ -2907,8 +2826,8 @@ int gfft_batch_plan_describe(gfft_batch_plan plan, char *buf, size_t len)
 {
     if (!plan || !buf || !len) return fail(GFFT_ERR_INVALID_ARG, "null argument");
     char head[256];
-    snprintf(head, sizeof(head), "batch jobs=%lld scratch_bytes=%zu offsets_even=%d branches=%d\n",
-             (long long)plan->count, plan->scratch_bytes, (int)plan->offsets_even, (int)plan->branches.size());
+    snprintf(head, sizeof(head), "batch jobs=%lld scratch_bytes=%zu offsets_even=%d\n", (long long)plan->count,
+             plan->scratch_bytes, (int)plan->offsets_even);
     return describe_launches(plan->launches, head, buf, len);
 }
 

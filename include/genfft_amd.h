@@ -258,9 +258,7 @@ GFFT_API int gfft_interleave_real_2d(int precision, void *out, int64_t out_strid
  * forward or (unscaled) inverse per directions[j].  Lengths up to 4096 (c64)
  * / 2048 (c128) run in one persistent launch that drains a device-side task
  * queue; longer lengths run as per-length batched launches.  Jobs must not
- * write where another job reads or writes (a job may be in place): in graph
- * mode the length classes run concurrently, on up to four branches of the
- * graph. */
+ * write where another job reads or writes (a job may be in place). */
 typedef struct gfft_batch_plan_s *gfft_batch_plan;
 
 #define GFFT_BATCH_GRAPH 0    /* capture the launches once into a HIP graph and replay it */
