@@ -8,8 +8,9 @@ value = GFLOP/s in the 5*N*log2(N) convention.
 The JSON line also carries
   roofline      -- the dominant kernel: algorithmic bytes per launch (one read
                    and one write of the elements it transforms) over its mean
-                   duration, measured live with HIP events recorded on the
-                   execution stream around every launch (gfft_plan_profile);
+                   duration, measured live with one HIP event pair on the
+                   execution stream around 20 back-to-back runs of each launch
+                   (gfft_plan_profile_launch);
                    `traffic` is the HBM bytes per launch measured by rocprofv3
                    PMC counters, read from profiles/pmc_traffic.json when present;
   e2e           -- the whole transform against the HBM roofline;
@@ -27,14 +28,19 @@ The JSON line also carries
                    fused forward + adjoint kernels against the same operator
                    through unfused launches (rfft / irfft plans + elementwise);
   uneven_batch  -- 20,000 complex64 transforms of sizes 2^6..2^16, mixed
-                   directions: one persistent launch replayed as a HIP graph
-                   against one plan launch per transform;
-  precision     -- N = 2^20 batch 64 with log-uniform magnitudes 1e-3..1e3:
-                   FP16 (+ per-transform scale), BF16 and FP32 modes, time and
-                   rel-L2 / max-abs against the GPU complex128 plan.
+                   directions: the batch plan (one persistent launch for the
+                   short lengths, per-class launches above) replayed as a HIP
+                   graph and run eagerly, against one plan launch per transform;
+  precision     -- N = 2^20 batch 64 and one 8192 x 8192 image with log-uniform
+                   magnitudes 1e-3..1e3: FP16 (+ per-transform scale), BF16 and
+                   FP32 modes, time and rel-L2 against the GPU complex128 plan
+                   and numpy's float64 FFT;
+  reference_cases -- genFFT's own benchmark cases (test/fft_bench.cpp): FFT_1D
+                   and RealFFT_1D latency at batch 1, n = 2..2^21, and FFT_Vert
+                   nfft x ncols, on device and host pointers.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-extras]
-                      [--leg fused_filter|uneven_batch|precision]
+                      [--leg fused_filter|uneven_batch|precision|reference_cases]
 """
 import argparse
 import json
@@ -89,7 +95,7 @@ def parse():
     ap.add_argument("--workload", default="c64_n2^20_b128", choices=sorted(WORKLOADS) + sorted(OTHER))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the fused-filter, uneven-batch and precision legs")
+                    help="skip the fused-filter, uneven-batch, precision and reference-case legs")
     ap.add_argument("--leg", choices=["fused_filter", "uneven_batch", "precision", "reference_cases"],
                     help="run only this leg and print its JSON object (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -480,6 +486,7 @@ def bench_precision(dev):
     N = 2^20 batch 64 (seed 20) and one 8192 x 8192 image (seed 21)."""
     import math as m_
 
+    import numpy as np
     import torch
 
     import genfft_amd as G
@@ -516,7 +523,17 @@ def bench_precision(dev):
                          "alg_gbs": round(2.0 * points * (8 if mode == "f32" else 4) / (ms * 1e-3) / 1e9, 1),
                          "rel_l2_vs_c128": float(torch.linalg.vector_norm(err) / torch.linalg.vector_norm(ref)),
                          "max_abs_vs_c128": float(err.abs().max())}
-            del x, xd, y, ref, yd, err, plan
+            # an independent reference: numpy's float64 FFT of the same stored input
+            # (transforms 0, batch/2 and batch-1 of the 1D batch; the whole 2D image)
+            if kind == C.GFFT_C2C:
+                rows = [0, batch // 2, batch - 1]
+                xs, ys = xd[rows].cpu().numpy(), yd[rows].cpu().numpy()
+                rn = np.fft.fft(xs, axis=-1)
+            else:
+                xs, ys = xd.cpu().numpy(), yd.cpu().numpy()
+                rn = np.fft.fft2(xs, axes=(-2, -1))
+            res[mode]["rel_l2_vs_numpy_f64"] = float(np.linalg.norm(ys - rn) / np.linalg.norm(rn))
+            del x, xd, y, ref, yd, err, plan, xs, ys, rn
         return res
 
     n, batch = 1 << 20, 64
@@ -525,7 +542,10 @@ def bench_precision(dev):
     out["2d_8192x8192"] = workload((1, side, side), 21, C.GFFT_C2C_2D, side, side, 1, G.fft2,
                                    5.0 * side * side * m_.log2(side * side), side * side)
     out["note"] = ("FP16/BF16 include the automatic per-transform scale pass (one extra read); "
-                   "errors vs the FP64 long-double oracle: tests/test_gpu_precision.py")
+                   "rel_l2_vs_c128: against this library's complex128 plan on the whole output; "
+                   "rel_l2_vs_numpy_f64: against numpy's float64 FFT of the same stored input (3 transforms "
+                   "of the 1D batch, the whole 2D image); against the long-double oracle: "
+                   "tests/test_gpu_precision.py")
     return out
 
 

@@ -34,3 +34,35 @@ int main() { use<float>(); use<double>(); return 0; }
     r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_header_accepts_the_reference_factory_spellings(tmp_path):
+    """genFFT's plug-in seam: the classes take a factory template parameter
+    (fft.h:47-56, :117, :173; FFTReal.h:188) and callers name the backends
+    (FFTBackend.h:40-70), e.g. the reference's test_x86.cpp forcing a tier.
+    Every spelling compiles and names this engine's factory."""
+    src = tmp_path / "f.cpp"
+    src.write_text(r'''
+#include <genFFT/fft.h>
+#include <genFFT/FFTReal.h>
+#include <type_traits>
+template <class T> void use() {
+    genfft::FFT<T, genfft::impl_generic::GetImpl> a(1024);
+    genfft::FFT<T, genfft::impl_native::GetImpl> b(1024);
+    genfft::FFT<T, genfft::impl_x86_dispatch::GetImpl> c(1024);
+    genfft::FFT<T, genfft::backend::GetImpl> d(1024);
+    genfft::FFTVert<T, genfft::impl_generic::GetVertImpl> v(64);
+    genfft::DIT<T, genfft::impl_x86_dispatch::GetDITImpl> t(64);
+    genfft::RealFFT<T, genfft::impl_native::GetImpl, genfft::impl_native::GetDITImpl> r(64);
+    genfft::FFTImplFactory<T> *f = genfft::impl_generic::GetImpl;
+    genfft::FFTVertImplFactory<T> *fv = genfft::backend::GetVertImpl;
+    genfft::FFTDITImplFactory<T> *fd = genfft::impl_native::GetDITImpl;
+    static_assert(std::is_same<decltype(f(8, T())), genfft::FFTImplPtr<T>>::value, "factory returns the impl pointer");
+    (void)a.size(); (void)b.size(); (void)c.size(); (void)d.size(); (void)v.size(); (void)t.size(); (void)r;
+    (void)f; (void)fv; (void)fd;
+}
+int main() { use<float>(); use<double>(); return 0; }
+''')
+    r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
