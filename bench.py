@@ -588,7 +588,7 @@ def main():
         plan = G.Plan(C.GFFT_C2C_2D, n, precision=0, n2=n2, batch=batch)
     elif kind == "c2r":
         # half spectra of float32 N(0,1) rows (bins 0 and n/2 real) -> n reals each, unscaled
-        x = torch.fft.rfft(torch.randn(batch, n, device=dev, generator=gen))
+        x = torch.fft.rfft(torch.randn(batch, n, device=dev, generator=gen)).contiguous()
         y = torch.empty(batch, n, device=dev)
         plan = G.Plan(C.GFFT_C2R, n, precision=0, batch=batch)
     elif kind == "r2c_2d":
@@ -598,7 +598,7 @@ def main():
         plan = G.Plan(C.GFFT_R2C_2D, n, precision=0, n2=n2, batch=batch)
     elif kind == "c2r_2d":
         # the n/2 + 1 non-redundant columns of a real image's spectrum -> the image (RealFFT2D::inverse)
-        x = torch.fft.rfft2(torch.randn(n2, n, device=dev, generator=gen))
+        x = torch.fft.rfft2(torch.randn(n2, n, device=dev, generator=gen)).contiguous()
         y = torch.empty(n2, n, device=dev)
         plan = G.Plan(C.GFFT_C2R_2D, n, precision=0, n2=n2, batch=batch, in_stride=n // 2 + 1, out_stride=n)
     else:

@@ -73,11 +73,30 @@ def _extent(t: torch.Tensor) -> int:
     return t.untyped_storage().nbytes() - t.storage_offset() * t.element_size()
 
 
+def _row_major(t: torch.Tensor) -> bool:
+    """Row-major up to padding: ignoring dimensions of size 1, the innermost
+    has stride 1 and each outer one a stride at least the span of the one
+    inside it.  The library reads and writes from data_ptr() with the plan's
+    own pitches, so a transposed or permuted view would be read as garbage."""
+    span, first = 1, True
+    for size, stride in zip(reversed(t.shape), reversed(t.stride())):
+        if size == 1:
+            continue
+        if (first and stride != 1) or stride < span:
+            return False
+        first = False
+        span = stride * size
+    return True
+
+
 def _check_gpu(*ts: torch.Tensor) -> torch.device:
     dev = None
     for t in ts:
         if not t.is_cuda:
             raise ValueError("genfft_amd works on GPU tensors only (got a CPU tensor)")
+        if not _row_major(t):
+            raise ValueError(f"tensor with strides {tuple(t.stride())} is not row-major (innermost stride 1, "
+                             "outer strides covering the inner ones); call .contiguous()")
         if dev is None:
             dev = t.device
         elif t.device != dev:

@@ -520,15 +520,23 @@ __device__ __forceinline__ void store_tile(const cx<T> *lds, const TileArgs &a, 
                     x0 = swap_ri(x0);
                     x1 = swap_ri(x1);
                 }
-                // streaming (nontemporal) stores: nothing in this launch
-                // reads the tile's output again.  Measured on c64 2^20 x 128:
-                // transposing row pass 0.392 -> 0.380 ms and column pass
-                // 0.441 -> 0.435 ms on one MI355X, no change on another.  (A
-                // runtime choice between this and a plain store is merged
-                // into one plain store by the compiler, which drops the hint.)
+                // Streaming (nontemporal) stores where each contiguous run
+                // of the tile covers whole 128-byte lines: nothing in this
+                // launch reads the output again.  Measured on c64 2^20 x 128
+                // (128-byte runs): transposing row pass 0.392 -> 0.380 ms,
+                // column pass 0.441 -> 0.435 ms on one MI355X, no change on
+                // another.  Narrower runs keep plain stores, so the L2 can
+                // merge neighbouring tiles' parts of a line: streamed, the
+                // 64-byte runs of c64 2^22 x 32 wrote 15-19% more than their
+                // bytes and the 32-byte runs of a 4096-point column pass 54%.
+                // (The choice is made at compile time: a runtime choice
+                // between the two stores of one value is merged by the
+                // compiler into one plain store.)
                 typedef float f4v __attribute__((ext_vector_type(4)));
                 const f4v q = {x0.re, x0.im, x1.re, x1.im};
-                __builtin_nontemporal_store(q, reinterpret_cast<f4v *>(p));
+                constexpr bool kStream = (OUTP == PAT_ROW ? L : M) * (int)sizeof(cx<T>) >= 128;
+                if constexpr (kStream) __builtin_nontemporal_store(q, reinterpret_cast<f4v *>(p));
+                else *reinterpret_cast<f4v *>(p) = q;
             }
             return;
         }

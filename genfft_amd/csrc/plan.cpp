@@ -2587,6 +2587,8 @@ int gfft_plan_profile(gfft_plan plan, const void *in, void *out, int direction, 
                       int max_launches, int *count)
 {
     if (!plan || !in || !out || !count) return fail(GFFT_ERR_INVALID_ARG, "null argument");
+    if ((direction != GFFT_FORWARD && direction != GFFT_INVERSE) || !(plan->dir_mask & (1 << direction)))
+        return fail(GFFT_ERR_INVALID_ARG, "direction %d is not one this plan runs", direction);
     if (int st = on_plan_device(plan->device)) return st;
     const int n = (int)plan->launches.size();
     *count = n;
@@ -2612,6 +2614,8 @@ int gfft_plan_profile_launch(gfft_plan plan, const void *in, void *out, int dire
     if (launch < 0 || launch >= (int)plan->launches.size())
         return fail(GFFT_ERR_INVALID_ARG, "launch %d out of range (plan has %zu)", launch, plan->launches.size());
     if (reps < 1 || reps > 10000) return fail(GFFT_ERR_INVALID_ARG, "reps %d out of range 1 .. 10000", reps);
+    if ((direction != GFFT_FORWARD && direction != GFFT_INVERSE) || !(plan->dir_mask & (1 << direction)))
+        return fail(GFFT_ERR_INVALID_ARG, "direction %d is not one this plan runs", direction);
     const Launch &l = plan->launches[launch];
     if (l.type == LT_MIX || l.type == LT_MIX_RESET)
         return fail(GFFT_ERR_UNSUPPORTED, "a work-queue launch runs once per reset");

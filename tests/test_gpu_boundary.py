@@ -245,3 +245,40 @@ def test_python_wrappers_reject_wrong_types_and_short_buffers(gpu):
     # a view into a larger buffer is accepted: the bytes past it belong to the storage
     big = torch.zeros(8 * n, dtype=torch.complex64, device=gpu)
     p.execute(big[n:], big[n:])
+
+
+def test_plans_reject_non_row_major_tensors(gpu):
+    """A plan reads and writes from the tensor's first element with its own
+    pitches, so the Python layer accepts row-major tensors (outer strides may
+    pad) and rejects transposed or permuted views instead of reading them as
+    garbage.  The half spectrum of a 2D real image, a padded view and its
+    dense copy give the same image."""
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    w, h = 64, 32
+    g = torch.Generator(device=gpu).manual_seed(9)
+    img = torch.randn(h, w, device=gpu, generator=g)
+    spec = torch.fft.rfft2(img)                                     # h x (w/2 + 1); ROCm's torch may return it
+    if not spec.is_contiguous():                                    # column-major, which a plan rejects
+        with pytest.raises(ValueError, match="row-major"):
+            G.Plan(C.GFFT_C2R_2D, w, precision=0, n2=h, in_stride=w // 2 + 1, out_stride=w).execute(
+                spec, torch.empty(h, w, device=gpu), inverse=True)
+    spec = spec.contiguous()
+    wide = torch.zeros(h, w, dtype=torch.complex64, device=gpu)
+    wide[:, : w // 2 + 1] = spec
+    view = wide[:, : w // 2 + 1]                                    # pitch w, row-major with padding
+    dense_plan = G.Plan(C.GFFT_C2R_2D, w, precision=0, n2=h, in_stride=w // 2 + 1, out_stride=w)
+    wide_plan = G.Plan(C.GFFT_C2R_2D, w, precision=0, n2=h, in_stride=w, out_stride=w)
+    a, b = torch.empty(h, w, device=gpu), torch.empty(h, w, device=gpu)
+    dense_plan.execute(spec, a, inverse=True)
+    wide_plan.execute(view, b, inverse=True)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    assert O.rel_l2(a.cpu().numpy() / (w * h), img.cpu().numpy()) <= 2 * O.tol_c64(w * h)
+    x = torch.randn(64, 32, dtype=torch.complex64, device=gpu, generator=g)
+    p = G.Plan(C.GFFT_C2C, 32, precision=0, batch=64)
+    y = torch.empty(64, 32, dtype=torch.complex64, device=gpu)
+    with pytest.raises(ValueError, match="row-major"):
+        p.execute(x.t().contiguous().t(), y)                         # column-major storage
+    with pytest.raises(ValueError, match="row-major"):
+        p.execute(x, torch.empty(32, 64, dtype=torch.complex64, device=gpu).t())

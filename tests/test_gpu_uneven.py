@@ -62,7 +62,7 @@ def test_uneven_all_lengths_all_jobs(gpu, prec):
 
 
 def test_uneven_graph_replay_matches_eager_and_per_job(gpu):
-    """The north-star config shape (lengths 2^6..2^16, 50/50 directions) at
+    """The uneven-batch configuration's shape (lengths 2^6..2^16, 50/50 directions) at
     reduced count; graph replayed several times with eager work in between."""
     import genfft_amd as G
     sizes, offs, inv, total = make_jobs(2000, 6, 16, seed=3)
