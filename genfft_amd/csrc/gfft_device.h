@@ -1057,13 +1057,21 @@ __device__ __forceinline__ bool fetch_tile_h(uint32_t (&reg)[M * L / NT], const 
     else in += s0;
     if constexpr (TOT % (4 * NT) == 0 && (INP == PAT_ROW ? L : M) % 4 == 0) {
         if (a.vec_in && nvalid == M) {
+            // Streaming (nontemporal) loads: each tile is read once, and the
+            // hint keeps the Infinity Cache for the intermediate the next
+            // pass reads and the L2 for merging the 64-byte output runs.
+            // Measured (FP16 2^20 x 64, 256 MiB intermediate): 0.422 ->
+            // 0.382-0.388 ms per step, the row pass writing 257.8 MiB per
+            // 256 instead of 269.9.  The complex64 tiles with narrow output
+            // runs got slower with it (c64 2^22 x 32: 1.080 -> 1.171 ms).
+            typedef unsigned u4v __attribute__((ext_vector_type(4)));
 #pragma unroll
             for (int it = 0; it < TOT / (4 * NT); it++) {
                 const int e = (tid + it * NT) * 4;
-                const uint4 *p;
-                if constexpr (INP == PAT_ROW) p = reinterpret_cast<const uint4 *>(in + (int64_t)(e / L) * a.in_inner + e % L);
-                else p = reinterpret_cast<const uint4 *>(in + (int64_t)(e / M) * a.in_elem + e % M);
-                const uint4 q = *p;
+                const u4v *p;
+                if constexpr (INP == PAT_ROW) p = reinterpret_cast<const u4v *>(in + (int64_t)(e / L) * a.in_inner + e % L);
+                else p = reinterpret_cast<const u4v *>(in + (int64_t)(e / M) * a.in_elem + e % M);
+                const u4v q = __builtin_nontemporal_load(p);
                 reg[4 * it] = q.x;
                 reg[4 * it + 1] = q.y;
                 reg[4 * it + 2] = q.z;
