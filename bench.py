@@ -456,28 +456,31 @@ def bench_reference_cases(dev):
             if inv_plan:
                 plan.execute(yd, zd, inverse=True)
         d = _time_ms(device, 50, 3) * 1e3
+        g = _device_ms(device, 50)
         del plan
-        return round(h, 2), round(d, 2), err
+        return round(h, 2), round(d, 2), (None if g is None else round(g * 1e3, 2)), err
 
     out = {"timing": "host_us: best of 5-20 synchronous calls on host buffers (H2D + transform + D2H, the "
-                     "genFFT drop-in's memory model); device_us: mean of 50 calls on device buffers, HIP events",
+                     "genFFT drop-in's memory model); device_us: mean of 50 back-to-back calls on device buffers "
+                     "from Python, HIP events (host launch cost included); gpu_us: the same 50 calls enqueued "
+                     "behind a spin kernel, i.e. the GPU's time alone",
            "FFT_1D": [], "RealFFT_1D": [], "FFT_Vert": []}
     for k in range(1, 22):
         n = 1 << k
         if k % 5 == 1:
             log(f"reference cases: FFT_1D / RealFFT_1D n = 2^{k}")
         x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex64)
-        h, d, err = case(C.GFFT_C2C, n, 0, x, np.empty_like(x), inv_plan=True)
-        out["FFT_1D"].append({"n": n, "host_us": h, "device_us": d, "err": err})
+        h, d, g, err = case(C.GFFT_C2C, n, 0, x, np.empty_like(x), inv_plan=True)
+        out["FFT_1D"].append({"n": n, "host_us": h, "device_us": d, "gpu_us": g, "err": err})
         r = rng.uniform(-1, 1, n).astype(np.float32)
-        h, d, _ = case(C.GFFT_R2C, n, 0, r, np.empty(n // 2 + 1, np.complex64))
-        out["RealFFT_1D"].append({"n": n, "host_us": h, "device_us": d})
+        h, d, g, _ = case(C.GFFT_R2C, n, 0, r, np.empty(n // 2 + 1, np.complex64))
+        out["RealFFT_1D"].append({"n": n, "host_us": h, "device_us": d, "gpu_us": g})
     for i, (nfft, ncols) in enumerate(_vert_args()):
         if i % 20 == 0:
             log(f"reference cases: FFT_Vert {nfft} x {ncols}")
         x = (rng.uniform(-1, 1, nfft * ncols) + 1j * rng.uniform(-1, 1, nfft * ncols)).astype(np.complex64)
-        h, d, err = case(C.GFFT_COLS, nfft, ncols, x, np.empty_like(x), inv_plan=True)
-        out["FFT_Vert"].append({"nfft": nfft, "ncols": ncols, "host_us": h, "device_us": d, "err": err})
+        h, d, g, err = case(C.GFFT_COLS, nfft, ncols, x, np.empty_like(x), inv_plan=True)
+        out["FFT_Vert"].append({"nfft": nfft, "ncols": ncols, "host_us": h, "device_us": d, "gpu_us": g, "err": err})
     return out
 
 

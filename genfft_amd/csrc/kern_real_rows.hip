@@ -110,8 +110,13 @@ constexpr RowsShape<T> rows_shape()
             {&real_rows_kernel<T, L, M, NT, RR_R2C, RS...>, &real_rows_kernel<T, L, M, NT, RR_C2R, RS...>}};
 }
 
-// the default c2c row-tile shapes of kern_tile_f32.hip / kern_tile_f64.hip
+// the default c2c row-tile shapes of kern_tile_f32.hip / kern_tile_f64.hip;
+// from L = 2 (n = 4) on, so short real transforms are one launch as well
+// (n = 2 runs as a copy and the unpack)
 const RowsShape<float> kRowsF32[] = {
+    rows_shape<float, 2, 1024, 128, 2>(),          rows_shape<float, 4, 1024, 256, 4>(),
+    rows_shape<float, 8, 512, 256, 8>(),           rows_shape<float, 16, 256, 256, 16>(),
+    rows_shape<float, 32, 128, 256, 8, 4>(),
     rows_shape<float, 64, 64, 256, 8, 8>(),        rows_shape<float, 128, 32, 256, 16, 8>(),
     rows_shape<float, 256, 16, 256, 16, 16>(),     rows_shape<float, 512, 8, 256, 8, 8, 8>(),
     rows_shape<float, 1024, 4, 256, 16, 16, 4>(),  rows_shape<float, 2048, 2, 256, 16, 16, 8>(),
@@ -119,6 +124,9 @@ const RowsShape<float> kRowsF32[] = {
     rows_shape<float, 16384, 1, 1024, 16, 16, 16, 4>(),
 };
 const RowsShape<double> kRowsF64[] = {
+    rows_shape<double, 2, 512, 64, 2>(),            rows_shape<double, 4, 512, 128, 4>(),
+    rows_shape<double, 8, 256, 128, 8>(),           rows_shape<double, 16, 128, 128, 16>(),
+    rows_shape<double, 32, 64, 128, 8, 4>(),
     rows_shape<double, 64, 32, 128, 8, 8>(),        rows_shape<double, 128, 16, 128, 16, 8>(),
     rows_shape<double, 256, 8, 256, 8, 8, 4>(),     rows_shape<double, 512, 4, 256, 8, 8, 8>(),
     rows_shape<double, 1024, 2, 256, 8, 4, 8, 4>(), rows_shape<double, 2048, 1, 256, 8, 8, 8, 4>(),

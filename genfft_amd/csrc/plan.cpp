@@ -909,10 +909,10 @@ struct Builder {
     }
 
     // Whether a real transform of length n runs in one row-tile pass (n/2 a
-    // single-pass row length from 64 on): unpack or pack fused in.
+    // single-pass row length from 2 on): unpack or pack fused in.
     bool rows_fusable(int64_t n) const
     {
-        return !store && opts.r2c_fuse && is_pow2(n) && n / 2 >= 64 && real_rows_entry(prec, (int)(n / 2));
+        return !store && opts.r2c_fuse && is_pow2(n) && n / 2 >= 2 && real_rows_entry(prec, (int)(n / 2));
     }
 
     // `rows` real transforms of length n in one launch: mode RR_R2C reads

@@ -401,8 +401,8 @@ def test_c2r_2d_plans_in_bounds(prec, w, h):
         if (prec == C.GFFT_F64 and w // 2 > 4096 * 2) or w // 2 > (1 << 22):
             continue
         assert st == C.GFFT_OK, (w, h, ins, outs, b, C.lib().gfft_last_error())
-        # rows of 128 .. 32768 (complex64) / 16384 (complex128) reals: pack fused into the row pass
-        fused = 128 <= w <= (32768 if prec == C.GFFT_F32 else 16384)
+        # rows of 4 .. 32768 (complex64) / 16384 (complex128) reals: pack fused into the row pass
+        fused = 4 <= w <= (32768 if prec == C.GFFT_F32 else 16384)
         assert ("real_rows mode=c2r" in text) == fused and ("pack n=" in text) != fused, (w, text)
 
 
@@ -423,7 +423,7 @@ def test_r2c_half_fuses_the_unpack_into_the_row_pass(prec, monkeypatch):
     for k in range(1, 24):
         n = 1 << k
         L = n // 2
-        rows = 64 <= L <= maxrow                     # one row-tile pass, unpack as its epilogue
+        rows = 2 <= L <= maxrow                      # one row-tile pass, unpack as its epilogue
         fused = L > maxrow and L // 512 <= 2048     # four-step, unpack in the last pass
         for od in (0, n // 2 + 2, n // 2 + 7):
             for b in (1, 4):
