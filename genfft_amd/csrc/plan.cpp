@@ -531,6 +531,9 @@ struct gfft_plan_s {
     void *stage = nullptr;  // copy of an input that overlaps the output (allocated on first use)
     std::mutex host_mu;
     void *hin = nullptr, *hout = nullptr;
+    // small host calls: pinned copies of both sides and a stream of their own
+    void *pin_in = nullptr, *pin_out = nullptr;
+    hipStream_t host_stream = nullptr;
     StreamOrder order;  // used when the plan has a workspace
     void *perm = nullptr;  // natural-order copy of a scrambled input (gfft_execute_scrambled)
 };
@@ -2436,6 +2439,9 @@ int gfft_plan_destroy(gfft_plan plan)
     if (plan->scratch) (void)hipFree(plan->scratch);
     if (plan->hin) (void)hipFree(plan->hin);
     if (plan->hout) (void)hipFree(plan->hout);
+    if (plan->pin_in) (void)hipHostFree(plan->pin_in);
+    if (plan->pin_out) (void)hipHostFree(plan->pin_out);
+    if (plan->host_stream) (void)hipStreamDestroy(plan->host_stream);
     if (plan->perm) (void)hipFree(plan->perm);
     if (plan->stage) (void)hipFree(plan->stage);
     delete plan;
@@ -2573,6 +2579,45 @@ int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)
         if (plan->hout) (void)hipFree(plan->hout);
         plan->hin = hin;
         plan->hout = hout;
+    }
+    // Small calls (up to 1 MiB a side) go through pinned copies on the plan's
+    // own stream with one synchronisation at the end: three synchronous
+    // copies from pageable memory cost more than the transform itself.
+    constexpr size_t kPinnedMax = (size_t)1 << 20;
+    if (plan->in_bytes <= kPinnedMax && plan->out_bytes <= kPinnedMax) {
+        if (!plan->pin_in || !plan->pin_out || !plan->host_stream) {
+            void *pi = nullptr, *po = nullptr;
+            hipStream_t hs = nullptr;
+            bool ok = hipHostMalloc(&pi, plan->in_bytes, hipHostMallocDefault) == hipSuccess &&
+                      hipHostMalloc(&po, plan->out_bytes, hipHostMallocDefault) == hipSuccess &&
+                      hipStreamCreateWithFlags(&hs, hipStreamNonBlocking) == hipSuccess;
+            if (!ok) {
+                if (pi) (void)hipHostFree(pi);
+                if (po) (void)hipHostFree(po);
+                return fail(GFFT_ERR_ALLOC, "pinned staging of %zu + %zu bytes", plan->in_bytes, plan->out_bytes);
+            }
+            if (plan->pin_in) (void)hipHostFree(plan->pin_in);
+            if (plan->pin_out) (void)hipHostFree(plan->pin_out);
+            if (plan->host_stream) (void)hipStreamDestroy(plan->host_stream);
+            plan->pin_in = pi;
+            plan->pin_out = po;
+            plan->host_stream = hs;
+        }
+        const hipStream_t hs = plan->host_stream;
+        memcpy(plan->pin_in, in, plan->in_bytes);
+        if (plan->out_has_gaps) memcpy(plan->pin_out, out, plan->out_bytes);
+        HIP_TRY(hipMemcpyAsync(plan->hin, plan->pin_in, plan->in_bytes, hipMemcpyHostToDevice, hs));
+        if (plan->out_has_gaps)
+            HIP_TRY(hipMemcpyAsync(plan->hout, plan->pin_out, plan->out_bytes, hipMemcpyHostToDevice, hs));
+        int st = gfft_execute(plan, plan->hin, plan->hout, direction, hs);
+        if (st) {
+            (void)hipStreamSynchronize(hs);
+            return st;
+        }
+        HIP_TRY(hipMemcpyAsync(plan->pin_out, plan->hout, plan->out_bytes, hipMemcpyDeviceToHost, hs));
+        HIP_TRY(hipStreamSynchronize(hs));
+        memcpy(out, plan->pin_out, plan->out_bytes);
+        return GFFT_OK;
     }
     HIP_TRY(hipMemcpy(plan->hin, in, plan->in_bytes, hipMemcpyHostToDevice));
     // keep the bytes the transform does not write (pitch gaps) intact
