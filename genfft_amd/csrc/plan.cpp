@@ -2606,16 +2606,21 @@ int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)
         const hipStream_t hs = plan->host_stream;
         memcpy(plan->pin_in, in, plan->in_bytes);
         if (plan->out_has_gaps) memcpy(plan->pin_out, out, plan->out_bytes);
-        HIP_TRY(hipMemcpyAsync(plan->hin, plan->pin_in, plan->in_bytes, hipMemcpyHostToDevice, hs));
-        if (plan->out_has_gaps)
-            HIP_TRY(hipMemcpyAsync(plan->hout, plan->pin_out, plan->out_bytes, hipMemcpyHostToDevice, hs));
-        int st = gfft_execute(plan, plan->hin, plan->hout, direction, hs);
-        if (st) {
+        // on any failure, wait for what was enqueued before returning: the
+        // pinned buffers are reused by the next call
+        auto bail = [&](int status) {
             (void)hipStreamSynchronize(hs);
-            return st;
-        }
-        HIP_TRY(hipMemcpyAsync(plan->pin_out, plan->hout, plan->out_bytes, hipMemcpyDeviceToHost, hs));
-        HIP_TRY(hipStreamSynchronize(hs));
+            return status;
+        };
+        hipError_t e = hipMemcpyAsync(plan->hin, plan->pin_in, plan->in_bytes, hipMemcpyHostToDevice, hs);
+        if (e == hipSuccess && plan->out_has_gaps)
+            e = hipMemcpyAsync(plan->hout, plan->pin_out, plan->out_bytes, hipMemcpyHostToDevice, hs);
+        if (e != hipSuccess) return bail(fail(GFFT_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e)));
+        if (int st = gfft_execute(plan, plan->hin, plan->hout, direction, hs)) return bail(st);
+        e = hipMemcpyAsync(plan->pin_out, plan->hout, plan->out_bytes, hipMemcpyDeviceToHost, hs);
+        if (e != hipSuccess) return bail(fail(GFFT_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e)));
+        e = hipStreamSynchronize(hs);
+        if (e != hipSuccess) return fail(GFFT_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
         memcpy(out, plan->pin_out, plan->out_bytes);
         return GFFT_OK;
     }
