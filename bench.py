@@ -91,7 +91,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c64_n2^20_b128", choices=sorted(WORKLOADS) + sorted(OTHER))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true",
@@ -243,11 +243,25 @@ def accuracy(kind, x, y, n, n2, prec, inv=False):
     return {"rel_l2_max": max(errs), "spot": f"transforms {rows}", "reference": "numpy.fft float64"}
 
 
+def _warm(fn, calls, min_ms=40.0):
+    """Untimed calls: at least `calls`, and at least min_ms of wall time with
+    the queue kept full.  An MI355X that was idle runs its first ~15-20 ms of
+    sustained work slower while its clocks come up (c64 2^14 x 8192: 0.60 ms
+    per call falling to 0.49 ms over the first 30 calls)."""
+    import torch
+    k, t0 = max(1, calls), time.perf_counter()
+    while True:
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        if (time.perf_counter() - t0) * 1e3 >= min_ms:
+            return
+        k *= 2
+
+
 def _time_ms(fn, reps, warm=2):
     import torch
-    for _ in range(warm):
-        fn()
-    torch.cuda.synchronize()
+    _warm(fn, warm)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
@@ -283,28 +297,34 @@ def pattern_copy_gbs(src, dst, n1, n2, m, mode, order):
 
 
 def _device_ms(fn, reps, warm=2):
-    """GPU time per call with host overhead hidden: a spin kernel holds the
-    stream while all reps are enqueued.  The start event must still be pending
-    when enqueueing ends (else the spin is lengthened and the run repeated);
-    None if the host never got ahead."""
+    """GPU time per call with host overhead hidden: copies of a 256 MiB buffer
+    (~0.1 ms each) hold the stream while all reps are enqueued.  Memory-bound
+    work keeps the clocks where a sustained load puts them; a one-wave spin
+    kernel lets them fall, and the calls after it measured 2-10% slower.  The
+    start event must still be pending when enqueueing ends (else the hold is
+    lengthened and the run repeated); None if the host never got ahead."""
     import torch
-    for _ in range(warm):
-        fn()
-    torch.cuda.synchronize()
-    cycles = 50_000_000
-    for _ in range(3):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda._sleep(cycles)
-        e0.record()
-        for _ in range(reps):
-            fn()
-        e1.record()
-        primed = not e0.query()
-        torch.cuda.synchronize()
-        if primed:
-            return e0.elapsed_time(e1) / reps
-        cycles *= 4
-    return None
+    _warm(fn, warm)
+    src = torch.empty(1 << 26, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    copies = 200
+    try:
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            for _ in range(copies):
+                dst.copy_(src)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            primed = not e0.query()
+            torch.cuda.synchronize()
+            if primed:
+                return e0.elapsed_time(e1) / reps
+            copies *= 4
+        return None
+    finally:
+        del src, dst
 
 
 def bench_filter(dev):
@@ -346,7 +366,7 @@ def bench_filter(dev):
                                                        res["fused"]["fwd_adj_device_ms"]) else
             round(res["unfused"]["fwd_adj_device_ms"] / res["fused"]["fwd_adj_device_ms"], 2),
             "timing": "*_ms: events around 20 calls, host launch and autograd-engine time included; "
-                      "*_device_ms: the same calls enqueued behind a spin kernel (device time only)",
+                      "*_device_ms: the same calls enqueued behind copies that hold the stream (device time only)",
             "speedup_fwd": round(res["unfused"]["fwd_ms"] / res["fused"]["fwd_ms"], 2),
             "speedup_fwd_adj_vs_torch_fft": round(res["unfused_torch_fft"]["fwd_adj_ms"] / res["fused"]["fwd_adj_ms"], 2),
             "baselines": "unfused: this library's rfft / irfft launches plus PyTorch elementwise ops; "
@@ -463,7 +483,7 @@ def bench_reference_cases(dev):
     out = {"timing": "host_us: best of 5-20 synchronous calls on host buffers (H2D + transform + D2H, the "
                      "genFFT drop-in's memory model); device_us: mean of 50 back-to-back calls on device buffers "
                      "from Python, HIP events (host launch cost included); gpu_us: the same 50 calls enqueued "
-                     "behind a spin kernel, i.e. the GPU's time alone",
+                     "behind copies that hold the stream, i.e. the GPU's time alone",
            "FFT_1D": [], "RealFFT_1D": [], "FFT_Vert": []}
     for k in range(1, 22):
         n = 1 << k

@@ -218,6 +218,11 @@ struct TileArgs {
     const void *pass_tw;  // Stockham pass twiddles, complex T
     const double *tw_lo;  // complex double [1 << tw_lo_bits]: W_Ntw^m
     const double *tw_hi;  // complex double [Ntw >> tw_lo_bits]: W_Ntw^(m << tw_lo_bits)
+    // Two-level FP64 table of W_L, b = (log2 L + 1) / 2: gen_lo[m] = W_L^m
+    // (m < 2^b), gen_hi[h] = W_L^(h << b), for pipelined kernels whose last
+    // pass builds its twiddles instead of reading pass_tw (PassTw::GEN)
+    const double *gen_lo;
+    const double *gen_hi;
     // Per-transform output scale (16-bit storage kernels): sequence starting
     // at element offset q of this launch's input belongs to transform
     // (scale_off + q) / scale_div, and its stored values are multiplied by
@@ -261,9 +266,17 @@ struct Tw4Pre {
     cx<double> hj, hd, hr;
 };
 
-template <typename T, int L, int M, int NT, class G, bool FASTJ, int NS, int R, bool LAST, bool PRE>
+// GEN (last pass only, NS * R == L): the pass builds W_L^(r*m) from the
+// two-level FP64 table `gen` (lo entries, then hi entries; see
+// TileArgs::gen_lo) instead of reading `tw`.  complex64: the base W_L^m is
+// one FP64 product, its powers an FP64 geometric progression (relative error
+// ~R * 2^-53), each rounded once to FP32, so the result matches the rounded
+// table entry except within ~1e-15 of a rounding boundary.  complex128: each
+// twiddle is its own FP64 product of two entries (within ~2 ulp).
+template <typename T, int L, int M, int NT, class G, bool FASTJ, int NS, int R, bool LAST, bool PRE, bool GEN = false>
 __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restrict__ tw, int tid,
-                                              const TileArgs &a, int s0, int o, const Tw4Pre &pre)
+                                              const TileArgs &a, int s0, int o, const Tw4Pre &pre,
+                                              const cx<double> *gen = nullptr)
 {
     constexpr int LR = L / R;
     constexpr int BF = M * LR;
@@ -288,8 +301,23 @@ __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restric
             for (int r = 0; r < R; r++) v[i][r] = lds[G::off(s, j + r * LR)];
         }
     }
-    cx<T> w[PER][R > 1 ? R - 1 : 1];
-    if constexpr (NS > 1) {
+    // GEN: the twiddles are built and applied one at a time after the
+    // barrier (no array of R - 1 of them stays live beside the prefetched
+    // tile); complex64 computes each butterfly's FP64 base here
+    constexpr int LB = (ilog2c(L) + 1) / 2;
+    const cx<double> *glo = gen, *ghi = gen + (1 << LB);
+    cx<double> gb[GEN && sizeof(T) == 4 ? PER : 1];
+    cx<T> w[GEN ? 1 : PER][R > 1 && !GEN ? R - 1 : 1];
+    if constexpr (GEN) {
+        static_assert(NS > 1 && NS * R == L, "generated twiddles serve the last pass");
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                const int m = jj[i] & (NS - 1);
+                gb[i] = cmul(glo[m & ((1 << LB) - 1)], ghi[m >> LB]);
+            }
+        }
+    } else if constexpr (NS > 1) {
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int m = jj[i] & (NS - 1);
@@ -304,7 +332,21 @@ __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restric
         if (BF % NT == 0 || b < BF) {
             const int s = sj[i], j = jj[i];
             const int m = j & (NS - 1);
-            if constexpr (NS > 1) {
+            if constexpr (GEN && sizeof(T) == 4) {
+                cx<double> wk = gb[i];
+                v[i][1] = cmul(v[i][1], cx<T>{(T)wk.re, (T)wk.im});
+#pragma unroll
+                for (int r = 2; r < R; r++) {
+                    wk = cmul(wk, gb[i]);
+                    v[i][r] = cmul(v[i][r], cx<T>{(T)wk.re, (T)wk.im});
+                }
+            } else if constexpr (GEN) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    const int e = r * m;
+                    v[i][r] = cmul(v[i][r], cmul(glo[e & ((1 << LB) - 1)], ghi[e >> LB]));
+                }
+            } else if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], w[i][r - 1]);
             }
@@ -354,16 +396,26 @@ __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restric
     __syncthreads();
 }
 
+// The passes of a radix sequence; with GEN the last one builds its twiddles
+// from `gen` (stockham_pass) and `tw` holds only the earlier passes' tables.
+template <typename T, int L, int M, int NT, class G, bool FASTJ, bool PRE, bool GEN, int NS, int R, int... RS>
+__device__ __forceinline__ void stockham_passes_g(cx<T> *lds, const cx<T> *__restrict__ tw, int tid,
+                                                  const TileArgs &a, int s0, int o, const Tw4Pre &pre,
+                                                  const cx<double> *gen)
+{
+    constexpr bool last = sizeof...(RS) == 0;
+    stockham_pass<T, L, M, NT, G, FASTJ, NS, R, last, PRE, last && GEN>(lds, tw, tid, a, s0, o, pre, gen);
+    if constexpr (!last) {
+        constexpr int used = NS > 1 ? NS * (R - 1) : 0;
+        stockham_passes_g<T, L, M, NT, G, FASTJ, PRE, GEN, NS * R, RS...>(lds, tw + used, tid, a, s0, o, pre, gen);
+    }
+}
+
 template <typename T, int L, int M, int NT, class G, bool FASTJ, bool PRE, int NS, int R, int... RS>
 __device__ __forceinline__ void stockham_passes(cx<T> *lds, const cx<T> *__restrict__ tw, int tid,
                                                 const TileArgs &a, int s0, int o, const Tw4Pre &pre)
 {
-    constexpr bool last = sizeof...(RS) == 0;
-    stockham_pass<T, L, M, NT, G, FASTJ, NS, R, last, PRE>(lds, tw, tid, a, s0, o, pre);
-    if constexpr (!last) {
-        constexpr int used = NS > 1 ? NS * (R - 1) : 0;
-        stockham_passes<T, L, M, NT, G, FASTJ, PRE, NS * R, RS...>(lds, tw + used, tid, a, s0, o, pre);
-    }
+    stockham_passes_g<T, L, M, NT, G, FASTJ, PRE, false, NS, R, RS...>(lds, tw, tid, a, s0, o, pre, nullptr);
 }
 
 template <int R, int... RS>
@@ -800,14 +852,34 @@ __global__ void __launch_bounds__(NT) tile_fft_kernel_h(TileArgs a)
 // of tile t+1 for the same reason (vector-memory counters drain in order).
 // ---------------------------------------------------------------------------
 // Pass twiddles of a pipelined tile are copied to LDS when the tile and the
-// table fit a CU's 160 KB together; otherwise (a single 16384-point
-// sequence: 136 KB of tile, 128 KB of table) the passes read them from
-// global memory, where the table stays in L2.
+// table fit a CU's 160 KB together.  Otherwise (a single 16384-point
+// complex64 sequence: 136 KB of tile, 128 KB of table, 120 KB of it the last
+// pass's) the last pass generates its twiddles (PassTw::GEN) from a two-level
+// FP64 table of W_L, 2^b + L/2^b entries (4 KB at L = 16384), kept in LDS with
+// the earlier passes' tables.  Reading them from global memory instead, each
+// wait on a twiddle load also waits on the next tile's prefetch issued before
+// it (the vector-memory counter drains in order), which undoes the pipeline.
 template <typename T, int ELEMS, int TWN>
 constexpr bool pass_tw_in_lds()
 {
     return (ELEMS + (TWN > 0 ? TWN : 1)) * sizeof(cx<T>) <= 160 * 1024;
 }
+
+// LDS layout of a pipelined tile's pass twiddles, in complex T units:
+// TWL every table; GEN the generation table (16-byte aligned, at offset 0)
+// then the tables of every pass but the last.
+template <typename T, int L, int ELEMS, int... RS>
+struct PassTw {
+    static constexpr int TWN = pass_tw_count<1, RS...>();
+    static constexpr int RL = last_radix<RS...>();
+    static constexpr int LAST = sizeof...(RS) > 1 ? L / RL * (RL - 1) : 0;
+    static constexpr int LB = (ilog2c(L) + 1) / 2;
+    static constexpr int GEN_D = (1 << LB) + (L >> LB);  // complex double entries
+    static constexpr int GEN_U = GEN_D * (int)(sizeof(cx<double>) / sizeof(cx<T>));
+    static constexpr bool TWL = pass_tw_in_lds<T, ELEMS, TWN>();
+    static constexpr bool GEN = !TWL && LAST > 0 && (ELEMS + 1 + GEN_U + TWN - LAST) * sizeof(cx<T>) <= 160 * 1024;
+    static constexpr int UNITS = TWL ? (TWN > 0 ? TWN : 1) : GEN ? GEN_U + (TWN - LAST > 0 ? TWN - LAST : 1) : 1;
+};
 
 // Thread index of a persistent loop, rebuilt on every iteration from the
 // wave's index (uniform, kept in an SGPR) and a lane id that v_mbcnt
@@ -890,32 +962,41 @@ __device__ __forceinline__ void put_tile(cx<T> *lds, const cx<T> (&reg)[M * L / 
 }
 
 // LDS footprint of a pipelined tile shape: the tile, then the pass twiddles
-// when both fit a CU's 160 KB.
+// (PassTw).
 template <typename T, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
 struct PipeLds {
     using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
-    static constexpr int TWN = pass_tw_count<1, RS...>();
-    static constexpr bool TWL = pass_tw_in_lds<T, G::ELEMS, TWN>();
+    using P = PassTw<T, L, G::ELEMS, RS...>;
     static constexpr int TILE = G::ELEMS;
-    static constexpr int TW = TWL && TWN > 0 ? TWN : 1;
+    static constexpr int TILE_A = (G::ELEMS + 1) & ~1;  // offset of the twiddles in one array (16-byte aligned)
+    static constexpr int TW = P::UNITS;
 };
 
 // The persistent loop over a launch's tiles, as workgroup `block` of `grid`
 // workgroups working on them (a kernel may hand other workgroups other work).
-// `lds` holds the tile and `twl` the pass twiddles (PipeLds).
+// `lds` holds the tile and `twl` (16-byte aligned) the pass twiddles (PipeLds).
 template <typename T, int L, int M, int NT, int LAYOUT, int INP, int OUTP, int... RS>
 __device__ __forceinline__ void pipe_tiles(const TileArgs &a, int block, int grid, cx<T> *lds, cx<T> *twl)
 {
     using G = TileGeom<L, M, LAYOUT, INP, OUTP>;
-    constexpr int TWN = pass_tw_count<1, RS...>();
+    using P = PassTw<T, L, G::ELEMS, RS...>;
     constexpr int E = M * L / NT;
     constexpr int LAST_R = last_radix<RS...>();
-    constexpr bool TWL = pass_tw_in_lds<T, G::ELEMS, TWN>();
     const int tid = threadIdx.x;
     const cx<T> *ptw = reinterpret_cast<const cx<T> *>(a.pass_tw);
-    if constexpr (TWL) {
-        for (int i = tid; i < TWN; i += NT) twl[i] = ptw[i];
+    const cx<double> *gen = nullptr;
+    if constexpr (P::TWL) {
+        for (int i = tid; i < P::TWN; i += NT) twl[i] = ptw[i];
         ptw = twl;
+    } else if constexpr (P::GEN) {
+        cx<double> *g = reinterpret_cast<cx<double> *>(twl);
+        const cx<double> *lo = reinterpret_cast<const cx<double> *>(a.gen_lo);
+        const cx<double> *hi = reinterpret_cast<const cx<double> *>(a.gen_hi);
+        constexpr int NLO = 1 << P::LB;
+        for (int i = tid; i < P::GEN_D; i += NT) g[i] = i < NLO ? lo[i] : hi[i - NLO];
+        for (int i = tid; i < P::TWN - P::LAST; i += NT) twl[P::GEN_U + i] = ptw[i];
+        ptw = twl + P::GEN_U;
+        gen = g;
     }
     const int slot = a.xcd_remap ? xcd_tile(block, grid) : block;
     auto decode = [&](int t, int &o2, int &o, int &s0, int &nv) {
@@ -987,8 +1068,8 @@ __device__ __forceinline__ void pipe_tiles(const TileArgs &a, int block, int gri
             decode(next, no2, no, ns0, nnv);
             vec = fetch_tile<T, L, M, NT, G, INP>(reg, a, no2, no, ns0, nnv, t);
         }
-        stockham_passes<T, L, M, NT, G, LAYOUT == LAY_SEQ, (LAYOUT == LAY_ELEM && NT % M == 0), 1, RS...>(
-            lds, ptw, t, a, s0, o, pre);
+        stockham_passes_g<T, L, M, NT, G, LAYOUT == LAY_SEQ, (LAYOUT == LAY_ELEM && NT % M == 0), P::GEN, 1, RS...>(
+            lds, ptw, t, a, s0, o, pre, gen);
         store_tile<T, L, M, NT, G, OUTP>(lds, a, o2, o, s0, nvalid, t);
         __syncthreads();
         tile = next;
@@ -1004,7 +1085,7 @@ __global__ void __launch_bounds__(NT) tile_fft_pipe_kernel(TileArgs a)
 {
     using S = PipeLds<T, L, M, NT, LAYOUT, INP, OUTP, RS...>;
     __shared__ cx<T> lds[S::TILE];
-    __shared__ cx<T> twl[S::TW];
+    __shared__ __align__(16) cx<T> twl[S::TW];
     pipe_tiles<T, L, M, NT, LAYOUT, INP, OUTP, RS...>(a, (int)blockIdx.x, (int)gridDim.x, lds, twl);
 }
 
@@ -1014,10 +1095,10 @@ struct PipeShape {
     using T = T_;
     static constexpr int NT = NT_;
     using S = PipeLds<T_, L, M, NT_, LAYOUT, INP, OUTP, RS...>;
-    static constexpr int ELEMS = S::TILE + S::TW;
+    static constexpr int ELEMS = S::TILE_A + S::TW;
     static __device__ __forceinline__ void run(const TileArgs &a, int block, int grid, cx<T_> *smem)
     {
-        pipe_tiles<T_, L, M, NT_, LAYOUT, INP, OUTP, RS...>(a, block, grid, smem, smem + S::TILE);
+        pipe_tiles<T_, L, M, NT_, LAYOUT, INP, OUTP, RS...>(a, block, grid, smem, smem + S::TILE_A);
     }
 };
 
@@ -1036,7 +1117,7 @@ __global__ void __launch_bounds__(A::NT) dual_pipe_kernel(TileArgs a, TileArgs b
     static_assert(sizeof(typename A::T) == sizeof(typename B::T), "one element type");
     using T = typename A::T;
     constexpr int E = A::ELEMS > B::ELEMS ? A::ELEMS : B::ELEMS;
-    __shared__ cx<T> smem[E];
+    __shared__ __align__(16) cx<T> smem[E];
     if ((int)blockIdx.x < ga) A::run(a, (int)blockIdx.x, ga, smem);
     else B::run(b, (int)blockIdx.x - ga, (int)gridDim.x - ga, smem);
 }

@@ -24,6 +24,7 @@ struct KernelEntry {
     int tune_only;   // 1: an alternative (radix order / tile width / threads) only the autotuner selects
     void (*fn)(TileArgs);
     int store;       // 0: complex T in memory; 1: FP16 pairs; 2: BF16 pairs (FP32 arithmetic)
+    int gen_tw;      // 1: the last pass builds its twiddles from TileArgs::gen_lo / gen_hi (W_L)
 };
 
 // Shape of a tile entry, without its kernel: each make_* below takes the
@@ -77,10 +78,9 @@ template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... R
 constexpr KernelEntry pipe_shape()
 {
     KernelEntry e = entry_shape<T, L, M, NT, LAY, INP, OUTP, RS...>();
-    using G = TileGeom<L, M, LAY, INP, OUTP>;
-    constexpr int twn = pass_tw_count<1, RS...>();
-    constexpr bool twl = pass_tw_in_lds<T, G::ELEMS, twn>();
-    e.lds_bytes = (int)((G::ELEMS + (twl ? (twn > 0 ? twn : 1) : 1)) * sizeof(cx<T>));
+    using S = PipeLds<T, L, M, NT, LAY, INP, OUTP, RS...>;
+    e.lds_bytes = (int)((S::TILE_A + S::TW) * sizeof(cx<T>));
+    e.gen_tw = S::P::GEN ? 1 : 0;
     e.persistent = 1;
     return e;
 }
@@ -109,7 +109,13 @@ template <class S, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
 constexpr KernelEntry make_pipe_entry_h()
 {
     KernelEntry e = pipe_shape<float, L, M, NT, LAY, INP, OUTP, RS...>();
-    e.lds_bytes += M * (int)sizeof(float);
+    // its own LDS: the tile, the pass tables when they fit beside the tile
+    // and the scales (else they are read from global memory), the scales
+    using G = TileGeom<L, M, LAY, INP, OUTP>;
+    constexpr int twn = pass_tw_count<1, RS...>();
+    constexpr bool twl = pass_tw_in_lds<float, G::ELEMS + M / 2, twn>();
+    e.lds_bytes = (int)((G::ELEMS + (twl && twn > 0 ? twn : 1)) * sizeof(cx<float>)) + M * (int)sizeof(float);
+    e.gen_tw = 0;
     e.store = std::is_same<S, st_f16>::value ? 1 : 2;
     e.fn = &tile_fft_pipe_kernel_h<S, L, M, NT, LAY, INP, OUTP, RS...>;
     return e;

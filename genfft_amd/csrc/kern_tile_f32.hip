@@ -67,9 +67,11 @@ const KernelEntry kTable[] = {
     // 0.479, 16*16*4: 0.493; the transposing row pass does not move with order)
     make_pipe_entry<T, 1024, 16, 1024, E_, C_, C_, 16, 4, 16>(),
     make_pipe_entry<T, 1024, 16, 1024, S_, R_, C_, 16, 16, 4>(),
-    // single 16384-point sequences, pass twiddles from global (tile: 136 KB of
-    // LDS); radix 16*4*16*16: c64 2^14 x 8192 0.500 ms (16*16*16*4: 0.509,
-    // 16*16*4*16: 0.502, 4*16*16*16: 0.521)
+    // single 16384-point sequences (tile: 136 KB of LDS); radix 16*4*16*16:
+    // c64 2^14 x 8192 0.500 ms with the pass twiddles read from global memory
+    // (16*16*16*4: 0.509, 16*16*4*16: 0.502, 4*16*16*16: 0.521), 0.442 ms with
+    // the last pass's twiddles generated from a 4 KB table in LDS
+    // (PassTw::GEN: the first three passes' 8 KB of tables fit beside it)
     make_pipe_entry<T, 16384, 1, 1024, S_, R_, R_, 16, 4, 16, 16>(),
     // --- autotuner alternatives: radix order, tile width, threads per tile ---
     make_variant<T, 256, 32, 512, S_, R_, R_, 16, 16>(),

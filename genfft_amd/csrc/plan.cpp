@@ -619,6 +619,12 @@ struct Builder {
             return fail(GFFT_ERR_UNSUPPORTED, "internal: output pattern does not match strides");
         int st = get_pass_twiddles(p->device, *k, &a.pass_tw, dry);
         if (st) return st;
+        if (k->gen_tw) {
+            TwTable g;
+            if ((st = get_two_level(p->device, k->L, &g, dry))) return st;
+            a.gen_lo = g.lo;
+            a.gen_hi = g.hi;
+        }
         // 16-byte vectors need even element offsets everywhere in the tile
         auto even = [](int64_t v) { return (v & 1) == 0; };
         if (prec == 0 && store == 0) {

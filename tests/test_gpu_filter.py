@@ -16,6 +16,8 @@ library's c2r.  Checks:
   * the fused path against the unfused one (library rfft / irfft launches
     plus elementwise ops) on the same inputs, and the fused path faster.
 """
+import time
+
 import numpy as np
 import pytest
 import torch
@@ -235,17 +237,24 @@ def test_fused_faster_than_unfused(gpu):
         H.grad = None
         ((fn(x, H) - t) ** 2).mean().backward()
 
+    # A GPU that sat idle runs its first ~20 ms of work at lower clocks: warm
+    # each form for 50 ms of calls, then keep the best of three timed rounds,
+    # the two forms taking turns.
     times = {}
     for fn in (G.spectral_filter, G.spectral_filter_plain):
-        for _ in range(3):
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 0.05:
             step(fn)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(10):
-            step(fn)
-        e1.record()
-        torch.cuda.synchronize()
-        times[fn.__name__] = e0.elapsed_time(e1) / 10
+            torch.cuda.synchronize()
+    for _ in range(3):
+        for fn in (G.spectral_filter, G.spectral_filter_plain):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(10):
+                step(fn)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 10
+            times[fn.__name__] = min(ms, times.get(fn.__name__, ms))
     print("fused vs unfused ms:", times)
     assert times["spectral_filter"] < times["spectral_filter_plain"]
