@@ -7,6 +7,7 @@
 #include <type_traits>
 
 #include "gfft_device.h"
+#include "gfft_wave.h"
 
 namespace gfft {
 
@@ -82,6 +83,19 @@ constexpr KernelEntry pipe_shape()
     e.lds_bytes = (int)((S::TILE_A + S::TW) * sizeof(cx<T>));
     e.gen_tw = S::P::GEN ? 1 : 0;
     e.persistent = 1;
+    return e;
+}
+
+// One wavefront per 1024-point complex64 row, no LDS (wave_rows1024_kernel),
+// radix stages (16, 16, 4); twiddles generated from the two-level table of W_1024.
+template <int NT>
+constexpr KernelEntry make_wave_entry()
+{
+    KernelEntry e = entry_shape<float, 1024, NT / 64, NT, LAY_SEQ, PAT_ROW, PAT_ROW, 16, 16, 4>();
+    e.lds_bytes = 0;
+    e.tune_only = 1;
+    e.gen_tw = 1;
+    e.fn = &wave_rows1024_kernel<NT>;
     return e;
 }
 

@@ -89,6 +89,8 @@ const KernelEntry kTable[] = {
     make_variant<T, 4096, 1, 512, S_, R_, R_, 8, 8, 8, 8>(),
     make_variant<T, 8192, 1, 512, S_, R_, R_, 16, 16, 16, 2>(),
     make_variant<T, 16384, 1, 1024, S_, R_, R_, 16, 16, 8, 8>(),
+    // one wavefront per 1024-point row, lanes exchanged by DPP / permlane swaps
+    make_wave_entry<256>(),
 };
 // The 2^20 four-step's two pipelined passes paired in one launch (column pass
 // of chunk k, transposing row pass of chunk k - 1).
