@@ -26,6 +26,7 @@ struct KernelEntry {
     void (*fn)(TileArgs);
     int store;       // 0: complex T in memory; 1: FP16 pairs; 2: BF16 pairs (FP32 arithmetic)
     int gen_tw;      // 1: the last pass builds its twiddles from TileArgs::gen_lo / gen_hi (W_L)
+    int wave;        // 1: one wavefront per row, lanes exchanged without LDS (wave_rows1024_kernel)
 };
 
 // Shape of a tile entry, without its kernel: each make_* below takes the
@@ -95,6 +96,7 @@ constexpr KernelEntry make_wave_entry()
     e.lds_bytes = 0;
     e.tune_only = 1;
     e.gen_tw = 1;
+    e.wave = 1;
     e.fn = &wave_rows1024_kernel<NT>;
     return e;
 }

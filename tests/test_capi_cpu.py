@@ -124,6 +124,21 @@ def test_tile_radix_orders(prec, n, col, row):
         assert [field(x, "rad") for x in t] == [col, row]
 
 
+def test_single_1024_row_takes_the_wave_kernel():
+    """One complex64 1024-point row runs on the one-wavefront kernel (no LDS,
+    radix stages 16, 16, 4); two rows, complex128 and 16-bit storage keep the
+    LDS tile kernels."""
+    def first_tile(**kw):
+        st, text = explain(desc(n=1024, **kw))
+        assert st == C.GFFT_OK, C.lib().gfft_last_error()
+        return tiles(text)[0]
+    t = first_tile(batch=1)
+    assert field(t, "lds") == "0" and field(t, "rad") == "16,16,4,0"
+    assert field(first_tile(batch=1, ind=1100, outd=1030), "lds") == "0"
+    for kw in (dict(batch=2), dict(batch=1, prec=C.GFFT_F64), dict(batch=1, prec=C.GFFT_F16)):
+        assert field(first_tile(**kw), "lds") != "0"
+
+
 def _random_descs(count, seed):
     """Random plan descriptors over every kind, precision, size class, batch
     and padded stride/distance the ABI accepts (and some it rejects)."""
