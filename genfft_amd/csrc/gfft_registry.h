@@ -27,6 +27,7 @@ struct KernelEntry {
     int store;       // 0: complex T in memory; 1: FP16 pairs; 2: BF16 pairs (FP32 arithmetic)
     int gen_tw;      // 1: the last pass builds its twiddles from TileArgs::gen_lo / gen_hi (W_L)
     int wave;        // 1: one wavefront per row, lanes exchanged without LDS (wave_rows1024_kernel)
+    int wide;        // 1: double-width row tile the planner takes for batches past the Infinity Cache
 };
 
 // Shape of a tile entry, without its kernel: each make_* below takes the
@@ -71,6 +72,16 @@ constexpr KernelEntry make_variant()
 {
     KernelEntry e = make_entry<T, L, M, NT, LAY, INP, OUTP, RS...>();
     e.tune_only = 1;
+    return e;
+}
+
+// A tuning alternative the planner also takes by itself for batches larger
+// than the Infinity Cache (KernelEntry::wide).
+template <typename T, int L, int M, int NT, int LAY, int INP, int OUTP, int... RS>
+constexpr KernelEntry make_wide_variant()
+{
+    KernelEntry e = make_variant<T, L, M, NT, LAY, INP, OUTP, RS...>();
+    e.wide = 1;
     return e;
 }
 

@@ -73,18 +73,23 @@ const KernelEntry kTable[] = {
     // the last pass's twiddles generated from a 4 KB table in LDS
     // (PassTw::GEN: the first three passes' 8 KB of tables fit beside it)
     make_pipe_entry<T, 16384, 1, 1024, S_, R_, R_, 16, 4, 16, 16>(),
-    // --- autotuner alternatives: radix order, tile width, threads per tile ---
-    make_variant<T, 256, 32, 512, S_, R_, R_, 16, 16>(),
+    // --- autotuner alternatives: radix order, tile width, threads per tile.
+    // The double-width row tiles (64 KB, 512 threads; make_wide_variant) are
+    // also the planner's choice for batches of >= 512 MiB: measured on MI355X
+    // through the autotuner, 1 GiB batches ran 3-5% faster on them (L = 256:
+    // 0.385 -> 0.372 ms, 512: 0.385 -> 0.365, 1024: 0.379 -> 0.365, 2048:
+    // 0.388 -> 0.367), 256 MiB batches (within the Infinity Cache) 0-12% slower ---
+    make_wide_variant<T, 256, 32, 512, S_, R_, R_, 16, 16>(),
     make_variant<T, 512, 8, 256, S_, R_, R_, 16, 8, 4>(),
-    make_variant<T, 512, 16, 512, S_, R_, R_, 8, 8, 8>(),
+    make_wide_variant<T, 512, 16, 512, S_, R_, R_, 8, 8, 8>(),
     make_variant<T, 1024, 4, 256, S_, R_, R_, 4, 16, 16>(),
     make_variant<T, 1024, 4, 256, S_, R_, R_, 8, 8, 16>(),
-    make_variant<T, 1024, 8, 512, S_, R_, R_, 16, 16, 4>(),
+    make_wide_variant<T, 1024, 8, 512, S_, R_, R_, 16, 16, 4>(),
     make_variant<T, 1024, 2, 128, S_, R_, R_, 16, 16, 4>(),
     make_variant<T, 1024, 4, 128, S_, R_, R_, 32, 32>(),   // two radix-32 passes, 32 elements per thread
     make_variant<T, 1024, 8, 256, S_, R_, R_, 32, 32>(),
     make_variant<T, 2048, 2, 256, S_, R_, R_, 8, 16, 16>(),
-    make_variant<T, 2048, 4, 512, S_, R_, R_, 16, 16, 8>(),
+    make_wide_variant<T, 2048, 4, 512, S_, R_, R_, 16, 16, 8>(),
     make_variant<T, 4096, 2, 512, S_, R_, R_, 16, 16, 16>(),
     make_variant<T, 4096, 1, 512, S_, R_, R_, 8, 8, 8, 8>(),
     make_variant<T, 8192, 1, 512, S_, R_, R_, 16, 16, 16, 2>(),

@@ -320,13 +320,14 @@ const KernelEntry *find_tile(int prec, int L, int inp, int outp, int prefer_m = 
     return best;
 }
 
-// Registry index of the one-wavefront row kernel of length L, or -1.
-int wave_variant(int prec, int L)
+// Registry index of the row -> row entry of length L with `flag` set
+// (KernelEntry::wave or ::wide), or -1.
+int row_variant_of(int prec, int L, int KernelEntry::*flag)
 {
     int cnt = 0;
     const KernelEntry *tab = tile_table(prec, 0, &cnt);
     for (int i = 0; i < cnt; i++)
-        if (tab[i].wave && tab[i].L == L) return i;
+        if (tab[i].*flag && tab[i].L == L && tab[i].inp == PAT_ROW && tab[i].outp == PAT_ROW) return i;
     return -1;
 }
 
@@ -708,11 +709,17 @@ struct Builder {
             ~Restore() { dst = v; }
         } restore{opts, saved};
         if (n <= maxrow) {
-            // a single row: the one-wavefront kernel where one is compiled for
-            // this length (no barrier; c64 n = 1024 on MI355X: 4.2 against
-            // 5.4 us for the LDS tile, which wins from 16 rows on)
-            const int variant = opts.row_variant >= 0 || batch != 1 || store ? opts.row_variant
-                                                                              : wave_variant(prec, (int)n);
+            // Without a tuned choice: a single row takes the one-wavefront
+            // kernel where one is compiled for this length (no barrier; c64
+            // n = 1024 on MI355X: 4.2 against 5.4 us for the LDS tile, which
+            // wins from 16 rows on), a batch of >= 512 MiB the double-width
+            // tile where one is registered (KernelEntry::wide).
+            int variant = opts.row_variant;
+            if (variant < 0 && !store) {
+                if (batch == 1) variant = row_variant_of(prec, (int)n, &KernelEntry::wave);
+                else if (n * batch * (int64_t)csize >= ((int64_t)512 << 20))
+                    variant = row_variant_of(prec, (int)n, &KernelEntry::wide);
+            }
             return tile((int)n, PAT_ROW, PAT_ROW, 0, in_buf, in_off, out_buf, out_off, batch, 1, 1, 0, 0,
                         in_dist, 1, 0, 0, out_dist, 1, TW_NONE, 0, first, last, variant);
         }

@@ -139,6 +139,24 @@ def test_single_1024_row_takes_the_wave_kernel():
         assert field(first_tile(**kw), "lds") != "0"
 
 
+@pytest.mark.parametrize("n,m", [(256, 16), (512, 8), (1024, 4), (2048, 2)])
+def test_batches_past_the_infinity_cache_take_the_wide_row_tile(n, m):
+    """complex64 single-pass batches of >= 512 MiB run on the double-width row
+    tile (512 threads, 64 KB of LDS), smaller ones and complex128 on the
+    default tile."""
+    def first_tile(**kw):
+        st, text = explain(desc(n=n, **kw))
+        assert st == C.GFFT_OK, C.lib().gfft_last_error()
+        return tiles(text)[0]
+    big, small = (1 << 30) // (8 * n), (256 << 20) // (8 * n)
+    t = first_tile(batch=big)
+    assert (field(t, "M"), field(t, "NT")) == (str(2 * m), "512")
+    t = first_tile(batch=small)
+    assert (field(t, "M"), field(t, "NT")) == (str(m), "256")
+    shape = lambda t: (field(t, "M"), field(t, "NT"), field(t, "rad"))
+    assert shape(first_tile(batch=big // 2, prec=C.GFFT_F64)) == shape(first_tile(batch=2, prec=C.GFFT_F64))
+
+
 def _random_descs(count, seed):
     """Random plan descriptors over every kind, precision, size class, batch
     and padded stride/distance the ABI accepts (and some it rejects)."""
