@@ -266,13 +266,12 @@ struct Tw4Pre {
     cx<double> hj, hd, hr;
 };
 
-// GEN (last pass only, NS * R == L): the pass builds W_L^(r*m) from the
-// two-level FP64 table `gen` (lo entries, then hi entries; see
-// TileArgs::gen_lo) instead of reading `tw`.  complex64: the base W_L^m is
-// one FP64 product, its powers an FP64 geometric progression (relative error
+// GEN (complex64, last pass only, NS * R == L): the pass builds W_L^(r*m)
+// from the two-level FP64 table `gen` (lo entries, then hi entries; see
+// TileArgs::gen_lo) instead of reading `tw`.  The base W_L^m is one FP64
+// product, its powers an FP64 geometric progression (relative error
 // ~R * 2^-53), each rounded once to FP32, so the result matches the rounded
-// table entry except within ~1e-15 of a rounding boundary.  complex128: each
-// twiddle is its own FP64 product of two entries (within ~2 ulp).
+// table entry except within ~1e-15 of a rounding boundary.
 template <typename T, int L, int M, int NT, class G, bool FASTJ, int NS, int R, bool LAST, bool PRE, bool GEN = false>
 __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restrict__ tw, int tid,
                                               const TileArgs &a, int s0, int o, const Tw4Pre &pre,
@@ -306,16 +305,15 @@ __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restric
     // tile); complex64 computes each butterfly's FP64 base here
     constexpr int LB = (ilog2c(L) + 1) / 2;
     const cx<double> *glo = gen, *ghi = gen + (1 << LB);
-    cx<double> gb[GEN && sizeof(T) == 4 ? PER : 1];
+    cx<double> gb[GEN ? PER : 1];
     cx<T> w[GEN ? 1 : PER][R > 1 && !GEN ? R - 1 : 1];
     if constexpr (GEN) {
         static_assert(NS > 1 && NS * R == L, "generated twiddles serve the last pass");
-        if constexpr (sizeof(T) == 4) {
+        static_assert(sizeof(T) == 4, "generated pass twiddles are a complex64 form");
 #pragma unroll
-            for (int i = 0; i < PER; i++) {
-                const int m = jj[i] & (NS - 1);
-                gb[i] = cmul(glo[m & ((1 << LB) - 1)], ghi[m >> LB]);
-            }
+        for (int i = 0; i < PER; i++) {
+            const int m = jj[i] & (NS - 1);
+            gb[i] = cmul(glo[m & ((1 << LB) - 1)], ghi[m >> LB]);
         }
     } else if constexpr (NS > 1) {
 #pragma unroll
@@ -332,19 +330,13 @@ __device__ __forceinline__ void stockham_pass(cx<T> *lds, const cx<T> *__restric
         if (BF % NT == 0 || b < BF) {
             const int s = sj[i], j = jj[i];
             const int m = j & (NS - 1);
-            if constexpr (GEN && sizeof(T) == 4) {
+            if constexpr (GEN) {
                 cx<double> wk = gb[i];
                 v[i][1] = cmul(v[i][1], cx<T>{(T)wk.re, (T)wk.im});
 #pragma unroll
                 for (int r = 2; r < R; r++) {
                     wk = cmul(wk, gb[i]);
                     v[i][r] = cmul(v[i][r], cx<T>{(T)wk.re, (T)wk.im});
-                }
-            } else if constexpr (GEN) {
-#pragma unroll
-                for (int r = 1; r < R; r++) {
-                    const int e = r * m;
-                    v[i][r] = cmul(v[i][r], cmul(glo[e & ((1 << LB) - 1)], ghi[e >> LB]));
                 }
             } else if constexpr (NS > 1) {
 #pragma unroll
@@ -877,7 +869,8 @@ struct PassTw {
     static constexpr int GEN_D = (1 << LB) + (L >> LB);  // complex double entries
     static constexpr int GEN_U = GEN_D * (int)(sizeof(cx<double>) / sizeof(cx<T>));
     static constexpr bool TWL = pass_tw_in_lds<T, ELEMS, TWN>();
-    static constexpr bool GEN = !TWL && LAST > 0 && (ELEMS + 1 + GEN_U + TWN - LAST) * sizeof(cx<T>) <= 160 * 1024;
+    static constexpr bool GEN = sizeof(T) == 4 && !TWL && LAST > 0 &&
+                                (ELEMS + 1 + GEN_U + TWN - LAST) * sizeof(cx<T>) <= 160 * 1024;
     static constexpr int UNITS = TWL ? (TWN > 0 ? TWN : 1) : GEN ? GEN_U + (TWN - LAST > 0 ? TWN - LAST : 1) : 1;
 };
 
