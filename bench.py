@@ -189,6 +189,17 @@ def cpu_baseline(x_cpu, n, batch, prec, gpu_out_cpu):
     flops = 5.0 * n * math.log2(n) * sample
     g = gpu_out_cpu[:sample].astype(np.complex128)
     agree = float(np.max(np.linalg.norm(g - ys, axis=1) / np.linalg.norm(ys, axis=1)))
+    batch1 = None
+    if kind == "reference":
+        # genFFT's own benchmark shape (test/fft_bench.cpp:41-100): one complex64
+        # transform per call, forward + inverse, one plan, on one thread; the
+        # reference_cases leg times the same calls on the GPU
+        lib.ref_c2c_pair_seconds.restype = ctypes.c_double
+        lib.ref_c2c_pair_seconds.argtypes = [ctypes.c_int] * 3
+        batch1 = []
+        for k in range(1, 22):
+            sec = lib.ref_c2c_pair_seconds(1 << k, max(1, int(3e6 / ((1 << k) * k))), 3)
+            batch1.append({"n": 1 << k, "us": round(sec * 1e6, 3)})
     return {
         "value": round(flops / best / 1e9, 3),
         "unit": "GFLOP/s",
@@ -200,6 +211,9 @@ def cpu_baseline(x_cpu, n, batch, prec, gpu_out_cpu):
                   f"this process may use: affinity mask and cgroup quota; the host has {host})",
         "seconds": round(best, 4),
         "gpu_vs_cpu_rel_l2_max": agree,
+        "batch1_fwd_inv_us": batch1,
+        "batch1_note": "genFFT, one complex64 transform per call, forward + unscaled inverse, one plan, one "
+                       "thread, best of 3 loops (the FFT_1D rows of reference_cases on the GPU)",
     }
 
 

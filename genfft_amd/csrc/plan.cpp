@@ -342,6 +342,18 @@ const DualEntry *find_dual(int prec, const KernelEntry *a, const KernelEntry *b)
     return nullptr;
 }
 
+// Narrowest tile (fewest sequences) registered for (L, inp, outp), tuning
+// alternatives aside; 0 if none.
+int min_m(int prec, int L, int inp, int outp, int store = 0)
+{
+    int cnt = 0, best = 0;
+    const KernelEntry *tab = tile_table(prec, store, &cnt);
+    for (int i = 0; i < cnt; i++)
+        if (tab[i].L == L && tab[i].inp == inp && tab[i].outp == outp && !tab[i].tune_only)
+            best = best ? std::min(best, tab[i].M) : tab[i].M;
+    return best;
+}
+
 int max_len(int prec, int inp, int outp, int store = 0)
 {
     int cnt = 0, best = 0;
@@ -742,7 +754,17 @@ struct Builder {
         int64_t chunk = std::max<int64_t>(1, chunk_bytes() / (n * (int64_t)csize));
         chunk = std::min(chunk, batch);
         need_scratch(chunk * n);
-        const int pm1 = opts.col_m, pm2 = opts.row_m;
+        // Without a tuned choice, a pass whose default tiles would leave CUs
+        // idle (small batches) takes the narrowest registered tile: more,
+        // shorter tiles run side by side.
+        auto narrow = [&](int L, int inp, int outp, int64_t seqs, int tuned) {
+            if (tuned || store) return tuned;
+            const KernelEntry *k = find_tile(prec, L, inp, outp, 0, opts.pipe_mask, -1, store);
+            const int m = min_m(prec, L, inp, outp, store);
+            return k && m < k->M && seqs / k->M < p->num_cus ? m : 0;
+        };
+        const int pm1 = narrow((int)N1, PAT_COL, PAT_COL, chunk * N2, opts.col_m);
+        const int pm2 = narrow((int)N2, PAT_ROW, PAT_COL, chunk * N1, opts.row_m);
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
             const int64_t cb = std::min(chunk, batch - b0);
             int st = tile((int)N1, PAT_COL, PAT_COL, pm1, in_buf, in_off + b0 * in_dist, B_SCR, 0, N2, cb, 1, 0,

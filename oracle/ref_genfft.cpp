@@ -16,6 +16,8 @@
 // except where the reference API uses scalars (RealFFT2D input stride).
 #include <genFFT/fft.h>
 
+#include <algorithm>
+#include <chrono>
 #include <complex>
 #include <cstdint>
 #include <thread>
@@ -85,6 +87,30 @@ REF_API void ref_c2c_batch_mt(void *out, const void *in, int n, int64_t batch, i
 {
     if (dbl) c2c_batch_mt((double *)out, (const double *)in, n, batch, inv, nthreads);
     else c2c_batch_mt((float *)out, (const float *)in, n, batch, inv, nthreads);
+}
+
+// Seconds per forward + unscaled inverse pair of one complex64 transform of
+// length n on this thread, as genFFT's own benchmark times it
+// (test/fft_bench.cpp:41-100: one plan, the same buffers, calls in a loop):
+// best of `trials` runs of `reps` pairs, after one untimed pair.
+REF_API double ref_c2c_pair_seconds(int n, int reps, int trials)
+{
+    std::vector<std::complex<float>> a((size_t)n), b((size_t)n);
+    for (int i = 0; i < n; i++) a[(size_t)i] = {(float)(i % 7) - 3.0f, (float)(i % 5) - 2.0f};
+    FFT<float> f(n);
+    f.transform<false>(b.data(), a.data());
+    f.transform<true>(a.data(), b.data());
+    double best = 1e30;
+    for (int t = 0; t < trials; t++) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps; r++) {
+            f.transform<false>(b.data(), a.data());
+            f.transform<true>(a.data(), b.data());
+        }
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        best = std::min(best, s / reps);
+    }
+    return best;
 }
 
 REF_API void ref_real_forward_batch_mt(void *out, int64_t out_dist, const void *in, int n, int64_t batch,

@@ -106,16 +106,16 @@ def test_repack_keeps_buffers_apart():
                    for ln in inner), text
 
 
-@pytest.mark.parametrize("prec,n,col,row", [
-    (C.GFFT_F32, 1 << 20, "16,4,16,0", "16,16,4,0"),   # measured fastest orders (MI355X)
-    (C.GFFT_F32, 1 << 22, "16,8,16,0", "16,16,8,0"),
-    (C.GFFT_F64, 1 << 22, "16,8,16,0", "16,16,8,0"),
-    (C.GFFT_F32, 1 << 14, None, "16,4,16,16"),
+@pytest.mark.parametrize("prec,n,batch,col,row", [
+    (C.GFFT_F32, 1 << 20, 128, "16,4,16,0", "16,16,4,0"),   # measured fastest orders (MI355X)
+    (C.GFFT_F32, 1 << 22, 32, "16,8,16,0", "16,16,8,0"),
+    (C.GFFT_F64, 1 << 22, 16, "16,8,16,0", "16,16,8,0"),
+    (C.GFFT_F32, 1 << 14, 8192, None, "16,4,16,16"),
 ])
-def test_tile_radix_orders(prec, n, col, row):
-    """The planner's kernels for the benchmark lengths carry the radix orders
+def test_tile_radix_orders(prec, n, batch, col, row):
+    """The planner's kernels for the benchmark shapes carry the radix orders
     their measurements chose (smallest radix in the middle of a column pass)."""
-    st, text = explain(desc(prec=prec, n=n, batch=2))
+    st, text = explain(desc(prec=prec, n=n, batch=batch))
     assert st == C.GFFT_OK, C.lib().gfft_last_error()
     t = tiles(text)
     if col is None:
@@ -155,6 +155,20 @@ def test_batches_past_the_infinity_cache_take_the_wide_row_tile(n, m):
     assert (field(t, "M"), field(t, "NT")) == (str(m), "256")
     shape = lambda t: (field(t, "M"), field(t, "NT"), field(t, "rad"))
     assert shape(first_tile(batch=big // 2, prec=C.GFFT_F64)) == shape(first_tile(batch=2, prec=C.GFFT_F64))
+
+
+@pytest.mark.parametrize("n,batch,col_m,row_m", [
+    (1 << 18, 1, "16", "8"),      # 16 row tiles of 16: narrow to 8
+    (1 << 20, 1, "8", "8"),       # 64 column and row tiles: both narrow
+    (1 << 20, 4, "16", "16"),     # 256 tiles: the default width
+    (1 << 20, 128, "16", "16"),
+])
+def test_small_four_step_batches_take_narrow_tiles(n, batch, col_m, row_m):
+    """A four-step pass whose default tiles would number fewer than the 256
+    CUs runs on the narrowest registered tile of its length."""
+    st, text = explain(desc(n=n, batch=batch))
+    assert st == C.GFFT_OK, C.lib().gfft_last_error()
+    assert [field(t, "M") for t in tiles(text)] == [col_m, row_m]
 
 
 def _random_descs(count, seed):
