@@ -385,7 +385,16 @@ def bench_filter(dev):
             "speedup_fwd_adj_vs_torch_fft": round(res["unfused_torch_fft"]["fwd_adj_ms"] / res["fused"]["fwd_adj_ms"], 2),
             "baselines": "unfused: this library's rfft / irfft launches plus PyTorch elementwise ops; "
                          "unfused_torch_fft: torch.fft.rfft / irfft (rocFFT) with PyTorch's own autograd",
-            "speedup_loss_step": round(res["unfused"]["loss_step_ms"] / res["fused"]["loss_step_ms"], 2)}
+            "speedup_loss_step": round(res["unfused"]["loss_step_ms"] / res["fused"]["loss_step_ms"], 2),
+            # forward: x in, y out; adjoint: x and gy in, gx and gH out (the gH partial rows are the
+            # fused path's own workspace, not algorithmic bytes)
+            "roofline_fwd_adj_device": None if res["fused"]["fwd_adj_device_ms"] is None else {
+                "alg_bytes": _FILTER_ALG_BYTES,
+                "achieved_gbs": round(_FILTER_ALG_BYTES / (res["fused"]["fwd_adj_device_ms"] * 1e-3) / 1e9, 1),
+                "frac": round(_FILTER_ALG_BYTES / (res["fused"]["fwd_adj_device_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+
+
+_FILTER_ALG_BYTES = 5 * 256 * 16384 * 4 + 3 * 8193 * 8  # x, y, x, gy, gx (float32); H twice, gH
 
 
 def bench_uneven(dev):
