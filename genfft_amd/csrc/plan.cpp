@@ -2639,8 +2639,9 @@ int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)
         if (!plan->pin_in || !plan->pin_out || !plan->host_stream) {
             void *pi = nullptr, *po = nullptr;
             hipStream_t hs = nullptr;
-            bool ok = hipHostMalloc(&pi, plan->in_bytes, hipHostMallocDefault) == hipSuccess &&
-                      hipHostMalloc(&po, plan->out_bytes, hipHostMallocDefault) == hipSuccess &&
+            // coherent: the kernels of the smallest calls read and write them in place
+            bool ok = hipHostMalloc(&pi, plan->in_bytes, hipHostMallocCoherent) == hipSuccess &&
+                      hipHostMalloc(&po, plan->out_bytes, hipHostMallocCoherent) == hipSuccess &&
                       hipStreamCreateWithFlags(&hs, hipStreamNonBlocking) == hipSuccess;
             if (!ok) {
                 if (pi) (void)hipHostFree(pi);
@@ -2663,6 +2664,21 @@ int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)
             (void)hipStreamSynchronize(hs);
             return status;
         };
+        // Up to 64 KiB a side the launches read and write the pinned buffers
+        // themselves (no copy commands; the transfers are a few PCIe round
+        // trips inside the kernel); above that, DMA copies on the same stream.
+        constexpr size_t kInPlaceMax = (size_t)64 << 10;
+        if (plan->in_bytes <= kInPlaceMax && plan->out_bytes <= kInPlaceMax) {
+            void *di = nullptr, *dout = nullptr;
+            hipError_t e = hipHostGetDevicePointer(&di, plan->pin_in, 0);
+            if (e == hipSuccess) e = hipHostGetDevicePointer(&dout, plan->pin_out, 0);
+            if (e != hipSuccess) return fail(GFFT_ERR_HIP, "hipHostGetDevicePointer: %s", hipGetErrorString(e));
+            if (int st = gfft_execute(plan, di, dout, direction, hs)) return bail(st);
+            e = hipStreamSynchronize(hs);
+            if (e != hipSuccess) return fail(GFFT_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+            memcpy(out, plan->pin_out, plan->out_bytes);
+            return GFFT_OK;
+        }
         hipError_t e = hipMemcpyAsync(plan->hin, plan->pin_in, plan->in_bytes, hipMemcpyHostToDevice, hs);
         if (e == hipSuccess && plan->out_has_gaps)
             e = hipMemcpyAsync(plan->hout, plan->pin_out, plan->out_bytes, hipMemcpyHostToDevice, hs);
