@@ -2617,29 +2617,17 @@ int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)
     if (!in || !out) return fail(GFFT_ERR_INVALID_ARG, "null buffer");
     if (int st = on_plan_device(plan->device)) return st;
     std::lock_guard<std::mutex> lk(plan->host_mu);
-    if (!plan->hin || !plan->hout) {
-        // both staging buffers or neither: a failed second allocation rolls back the first
-        void *hin = nullptr, *hout = nullptr;
-        if (hipMalloc(&hin, plan->in_bytes) != hipSuccess)
-            return fail(GFFT_ERR_ALLOC, "device staging of %zu bytes", plan->in_bytes);
-        if (hipMalloc(&hout, plan->out_bytes) != hipSuccess) {
-            (void)hipFree(hin);
-            return fail(GFFT_ERR_ALLOC, "device staging of %zu bytes", plan->out_bytes);
-        }
-        if (plan->hin) (void)hipFree(plan->hin);
-        if (plan->hout) (void)hipFree(plan->hout);
-        plan->hin = hin;
-        plan->hout = hout;
-    }
-    // Small calls (up to 1 MiB a side) go through pinned copies on the plan's
-    // own stream with one synchronisation at the end: three synchronous
-    // copies from pageable memory cost more than the transform itself.
+    // Calls up to 1 MiB a side: the launches read and write plan-owned pinned
+    // (coherent) host buffers in place, on the plan's own stream, with one
+    // synchronisation and no copy commands.  Measured on MI355X, c64 forward +
+    // inverse pairs: n = 2^14 98 -> 71 us, 2^17 270 -> 250 us against DMA
+    // copies through the same buffers; from 2 MiB on the runtime's pageable
+    // copies were as fast or faster (2^20: 1.57 against 1.68 ms in place).
     constexpr size_t kPinnedMax = (size_t)1 << 20;
     if (plan->in_bytes <= kPinnedMax && plan->out_bytes <= kPinnedMax) {
         if (!plan->pin_in || !plan->pin_out || !plan->host_stream) {
             void *pi = nullptr, *po = nullptr;
             hipStream_t hs = nullptr;
-            // coherent: the kernels of the smallest calls read and write them in place
             bool ok = hipHostMalloc(&pi, plan->in_bytes, hipHostMallocCoherent) == hipSuccess &&
                       hipHostMalloc(&po, plan->out_bytes, hipHostMallocCoherent) == hipSuccess &&
                       hipStreamCreateWithFlags(&hs, hipStreamNonBlocking) == hipSuccess;
@@ -2656,40 +2644,34 @@ int gfft_execute_host(gfft_plan plan, const void *in, void *out, int direction)
             plan->host_stream = hs;
         }
         const hipStream_t hs = plan->host_stream;
+        void *di = nullptr, *dout = nullptr;
+        hipError_t e = hipHostGetDevicePointer(&di, plan->pin_in, 0);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&dout, plan->pin_out, 0);
+        if (e != hipSuccess) return fail(GFFT_ERR_HIP, "hipHostGetDevicePointer: %s", hipGetErrorString(e));
         memcpy(plan->pin_in, in, plan->in_bytes);
-        if (plan->out_has_gaps) memcpy(plan->pin_out, out, plan->out_bytes);
-        // on any failure, wait for what was enqueued before returning: the
-        // pinned buffers are reused by the next call
-        auto bail = [&](int status) {
-            (void)hipStreamSynchronize(hs);
-            return status;
-        };
-        // Up to 64 KiB a side the launches read and write the pinned buffers
-        // themselves (no copy commands; the transfers are a few PCIe round
-        // trips inside the kernel); above that, DMA copies on the same stream.
-        constexpr size_t kInPlaceMax = (size_t)64 << 10;
-        if (plan->in_bytes <= kInPlaceMax && plan->out_bytes <= kInPlaceMax) {
-            void *di = nullptr, *dout = nullptr;
-            hipError_t e = hipHostGetDevicePointer(&di, plan->pin_in, 0);
-            if (e == hipSuccess) e = hipHostGetDevicePointer(&dout, plan->pin_out, 0);
-            if (e != hipSuccess) return fail(GFFT_ERR_HIP, "hipHostGetDevicePointer: %s", hipGetErrorString(e));
-            if (int st = gfft_execute(plan, di, dout, direction, hs)) return bail(st);
-            e = hipStreamSynchronize(hs);
-            if (e != hipSuccess) return fail(GFFT_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
-            memcpy(out, plan->pin_out, plan->out_bytes);
-            return GFFT_OK;
-        }
-        hipError_t e = hipMemcpyAsync(plan->hin, plan->pin_in, plan->in_bytes, hipMemcpyHostToDevice, hs);
-        if (e == hipSuccess && plan->out_has_gaps)
-            e = hipMemcpyAsync(plan->hout, plan->pin_out, plan->out_bytes, hipMemcpyHostToDevice, hs);
-        if (e != hipSuccess) return bail(fail(GFFT_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e)));
-        if (int st = gfft_execute(plan, plan->hin, plan->hout, direction, hs)) return bail(st);
-        e = hipMemcpyAsync(plan->pin_out, plan->hout, plan->out_bytes, hipMemcpyDeviceToHost, hs);
-        if (e != hipSuccess) return bail(fail(GFFT_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e)));
+        if (plan->out_has_gaps) memcpy(plan->pin_out, out, plan->out_bytes);  // the bytes no launch writes
+        int st = gfft_execute(plan, di, dout, direction, hs);
+        // wait for whatever was enqueued, also on failure: the next call reuses the buffers
         e = hipStreamSynchronize(hs);
+        if (st) return st;
         if (e != hipSuccess) return fail(GFFT_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
         memcpy(out, plan->pin_out, plan->out_bytes);
         return GFFT_OK;
+    }
+    // larger calls: the runtime's copies through device staging buffers
+    if (!plan->hin || !plan->hout) {
+        // both staging buffers or neither: a failed second allocation rolls back the first
+        void *hin = nullptr, *hout = nullptr;
+        if (hipMalloc(&hin, plan->in_bytes) != hipSuccess)
+            return fail(GFFT_ERR_ALLOC, "device staging of %zu bytes", plan->in_bytes);
+        if (hipMalloc(&hout, plan->out_bytes) != hipSuccess) {
+            (void)hipFree(hin);
+            return fail(GFFT_ERR_ALLOC, "device staging of %zu bytes", plan->out_bytes);
+        }
+        if (plan->hin) (void)hipFree(plan->hin);
+        if (plan->hout) (void)hipFree(plan->hout);
+        plan->hin = hin;
+        plan->hout = hout;
     }
     HIP_TRY(hipMemcpy(plan->hin, in, plan->in_bytes, hipMemcpyHostToDevice));
     // keep the bytes the transform does not write (pitch gaps) intact

@@ -1,9 +1,8 @@
 """Host-pointer execution (gfft_execute_host), the memory model of genFFT's
 callers (include/genFFT/fft.h:80-85 on host arrays): the call returns with the
-result in `out`.  Three transfer paths, chosen by the call's size a side:
-  * up to 64 KiB: the launches read and write the plan's pinned buffers in place;
-  * up to 1 MiB: DMA copies through the pinned buffers on the plan's stream;
-  * above: the runtime's pageable copies.
+result in `out`.  Two transfer paths, chosen by the call's size a side:
+  * up to 1 MiB: the launches read and write the plan's pinned buffers in place;
+  * above: the runtime's pageable copies through device staging buffers.
 Each path against the long-double oracle, forward and unscaled inverse, with
 pitch gaps in the output left untouched, and calls of one plan repeated.
 """
@@ -29,7 +28,7 @@ def _exec_host(plan, x, y, inv=False):
     C.check(C.lib().gfft_execute_host(plan._h, P(x), P(y), C.GFFT_INVERSE if inv else C.GFFT_FORWARD))
 
 
-# bytes a side: 64 B, 8 KiB, 64 KiB (in place); 128 KiB, 1 MiB (DMA); 2 MiB (pageable)
+# bytes a side: 64 B, 8 KiB, 64 KiB, 128 KiB, 1 MiB (in place); 2 MiB (pageable)
 @pytest.mark.parametrize("n,batch", [(8, 1), (1024, 1), (4096, 2), (16384, 1), (1 << 17, 1), (1 << 18, 1)])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_host_pointers_vs_oracle(gpu, n, batch, prec):
