@@ -263,7 +263,17 @@ __global__ void __launch_bounds__(256) amax_kernel(AmaxArgs a)
     const int64_t e0 = c * a.chunk, e1 = e0 + a.chunk < E ? e0 + a.chunk : E;
     const uint32_t *base = in + b * a.dist;
     float m = 0.0f;
-    if (a.vec) {  // width, pitch, dist multiples of 4 and 16-byte aligned input
+    if (a.vec && (a.rows == 1 || a.pitch == a.width)) {
+        // a contiguous transform: 16-byte loads straight from the chunk, no
+        // per-load 64-bit row division, several loads in flight per thread
+        const uint4 *q = reinterpret_cast<const uint4 *>(base + e0);
+        const int nq = (int)((e1 - e0) >> 2);
+#pragma unroll 8
+        for (int i = threadIdx.x; i < nq; i += 256) {
+            const uint4 v = q[i];
+            m = fmaxf(m, fmaxf(fmaxf(l1_of<S>(v.x), l1_of<S>(v.y)), fmaxf(l1_of<S>(v.z), l1_of<S>(v.w))));
+        }
+    } else if (a.vec) {  // width, pitch, dist multiples of 4 and 16-byte aligned input
         for (int64_t e = e0 + 4 * (int64_t)threadIdx.x; e < e1; e += 4 * 256) {
             const int64_t r = e / a.width, w = e - r * a.width;
             const uint4 q = *reinterpret_cast<const uint4 *>(base + r * a.pitch + w);
@@ -280,7 +290,13 @@ __global__ void __launch_bounds__(256) amax_kernel(AmaxArgs a)
     __syncthreads();
     if (threadIdx.x == 0) {
         m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        atomicMax(a.bits + b, __float_as_uint(m));
+        // the running max only grows: a block whose max is not above a value
+        // already seen (possibly stale, never above the true running max)
+        // cannot change it and skips the atomic, which the blocks of one
+        // transform would otherwise queue on one cache line
+        const uint32_t mb = __float_as_uint(m);
+        if (mb > __hip_atomic_load(a.bits + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            atomicMax(a.bits + b, mb);
     }
 }
 
@@ -327,7 +343,10 @@ hipError_t launch_amax_scale(AmaxArgs a, hipStream_t s)
         else hipLaunchKernelGGL(amax_small_kernel<st_bf16>, dim3(g), dim3(256), 0, s, a);
         return hipGetLastError();
     }
-    a.chunk = 8192;
+    // 64 Ki elements (256 KiB) per block: with 8 Ki, FP16 2^20 x 64 with the
+    // automatic scale took 0.450 ms and one 8192 x 8192 image 0.551 ms; with
+    // 64 Ki and the skipped atomics (amax_kernel) 0.450 and 0.517 ms (MI355X)
+    a.chunk = 65536;
     a.chunks = (E + a.chunk - 1) / a.chunk;
     if (a.batch * a.chunks > 0x7fffffffll) return hipErrorInvalidValue;
     a.vec = (a.width % 4 == 0) && (a.pitch % 4 == 0) && (a.dist % 4 == 0) && (((uintptr_t)a.in & 15) == 0);
