@@ -96,7 +96,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the fused-filter, uneven-batch, precision and reference-case legs")
-    ap.add_argument("--leg", choices=["fused_filter", "uneven_batch", "precision", "reference_cases"],
+    ap.add_argument("--leg", choices=["fused_filter", "uneven_batch", "precision", "reference_cases", "tile_widths"],
                     help="run only this leg and print its JSON object (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (scale reference for PMC byte counters)")
@@ -527,6 +527,52 @@ def bench_reference_cases(dev):
     return out
 
 
+def bench_tile_widths(dev):
+    """The single-pass kernels the autotuner can pick for complex64 rows of
+    1024 points (the LDS tiles of 2-8 rows and 128-512 threads, and the
+    one-wavefront kernel with no LDS), each timed by gfft_autotune at batches
+    from one row to 1 GiB; and which one the planner takes by default."""
+    import ctypes
+
+    import torch
+
+    import genfft_amd as G
+    from genfft_amd import _capi as C
+    out = []
+    for b in (1, 16, 256, 2048, 8192, 32768, 131072):
+        C.check(C.lib().gfft_tuning_clear())
+        G.clear_plan_cache()
+        plan = G.Plan(C.GFFT_C2C, 1024, batch=b)
+        default = plan.describe().splitlines()[1]
+        x = torch.zeros(b, 1024, dtype=torch.complex64, device=dev)
+        y = torch.empty_like(x)
+        _warm(lambda: plan.execute(x, y), 2)  # the clocks up before the candidates are timed
+        del plan, x, y
+        rep = G.autotune(1024, b, torch.complex64, reps=50 if b < 8192 else 20)
+        rows = []
+        for c in sorted(rep["candidates"], key=lambda c: c["ms"]):
+            d = C.GfftDesc(C.GFFT_C2C, 0, 1024, 0, b, 0, 0, 0, 0, 0, 0)
+            # the candidate's kernel, as the planner describes it under that tuning entry
+            path = os.path.join("/tmp", f"gfft_tile_widths_{os.getpid()}.json")
+            with open(path, "w") as f:
+                f.write('{"version": 1, "entries": [\n {"precision": 0, "n": 1024, "n1_log2": 0, "col_m": 0, '
+                        f'"row_m": 0, "pipe_mask": 3, "row_variant": {c["row_variant"]}}}\n]}}\n')
+            C.check(C.lib().gfft_tuning_clear())
+            G.load_tuning(path)
+            os.remove(path)
+            buf = ctypes.create_string_buffer(1 << 14)
+            C.check(C.lib().gfft_plan_explain(ctypes.byref(d), buf, len(buf)))
+            line = buf.value.decode().splitlines()[1]
+            shape = " ".join(w for w in line.split() if w.split("=")[0] in ("M", "NT", "lds", "rad"))
+            rows.append({"us": round(c["ms"] * 1e3, 2), "kernel": shape})
+        C.check(C.lib().gfft_tuning_clear())
+        out.append({"batch": b, "default": " ".join(w for w in default.split()
+                                                     if w.split("=")[0] in ("M", "NT", "lds", "rad")),
+                    "candidates": rows})
+    return {"config": "complex64, n = 1024, forward; each candidate timed by gfft_autotune (events around "
+                      "repeated calls after two untimed ones)", "batches": out}
+
+
 def bench_precision(dev):
     """Log-uniform |x| in 1e-3..1e3, uniform phase, in FP16 + scale, BF16 and FP32:
     N = 2^20 batch 64 (seed 20) and one 8192 x 8192 image (seed 21)."""
@@ -609,7 +655,7 @@ def main():
 
     if a.leg:
         leg = {"fused_filter": bench_filter, "uneven_batch": bench_uneven, "precision": bench_precision,
-               "reference_cases": bench_reference_cases}[a.leg]
+               "reference_cases": bench_reference_cases, "tile_widths": bench_tile_widths}[a.leg]
         print(json.dumps({a.leg: leg(torch.device("cuda", 0))}), flush=True)
         return
     kind, n2 = "c2c", 0
