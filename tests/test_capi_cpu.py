@@ -139,6 +139,16 @@ def test_single_1024_row_takes_the_wave_kernel():
         assert field(first_tile(**kw), "lds") != "0"
 
 
+@pytest.mark.parametrize("batch,m,nt", [(16, "2", "128"), (255, "2", "128"), (1024, "4", "256"), (8192, "4", "256")])
+def test_small_row_batches_take_the_narrow_tile(batch, m, nt):
+    """complex64 1024-point rows: while the default 4-row tiles would number
+    fewer than the 256 CUs, the 2-row tile of 128 threads runs instead."""
+    st, text = explain(desc(n=1024, batch=batch))
+    assert st == C.GFFT_OK, C.lib().gfft_last_error()
+    t = tiles(text)[0]
+    assert (field(t, "M"), field(t, "NT")) == (m, nt)
+
+
 @pytest.mark.parametrize("n,m", [(256, 16), (512, 8), (1024, 4), (2048, 2)])
 def test_batches_past_the_infinity_cache_take_the_wide_row_tile(n, m):
     """complex64 single-pass batches of >= 512 MiB run on the double-width row
