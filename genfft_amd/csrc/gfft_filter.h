@@ -36,8 +36,9 @@ struct ReduceArgs {
     int64_t bins, rows, pitch, seg, out_pitch;
 };
 
-// which: 0 forward, 1 adjoint (gx + partials), 2 partial reduction (gH)
-hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s);
+// which: 0 forward, 1 adjoint (gx + partials), 2 partial reduction (gH);
+// `stop`, if given, completes with the launch's last kernel
+hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s, hipEvent_t stop = nullptr);
 // threads and radices (of the forward FFT, or with `inverse` of the inverse
 // FFT) of the fused kernel for L = n/2; 0 if unsupported
 int filter_shape(int prec, int L, int *nt, int *rad, int inverse);

@@ -607,7 +607,7 @@ int filter_shape(int prec, int L, int *nt, int *rad, int inverse)
     return prec ? take(lookup(kF64, L)) : take(lookup(kF32, L));
 }
 
-hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s)
+hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s, hipEvent_t stop)
 {
     const int L = (int)(a.bins - 1);
     void (*fn)(FilterArgs) = nullptr;
@@ -638,9 +638,9 @@ hipError_t launch_filter(int prec, int which, const FilterArgs &a, hipStream_t s
         }
         r.dst = a.y;
         r.out_pitch = 0;
-        if (prec) hipLaunchKernelGGL(filter_reduce_kernel<double>, dim3((unsigned)gx, 1), dim3(32 * kReduceLanes), 0, s, r);
-        else hipLaunchKernelGGL(filter_reduce_kernel<float>, dim3((unsigned)gx, 1), dim3(32 * kReduceLanes), 0, s, r);
-        return hipGetLastError();
+        const void *fr = prec ? (const void *)&filter_reduce_kernel<double> : (const void *)&filter_reduce_kernel<float>;
+        void *rargs[] = {&r};
+        return hipExtLaunchKernel(fr, dim3((unsigned)gx, 1), dim3(32 * kReduceLanes), rargs, 0, s, nullptr, stop, 0);
     }
     if (prec) {
         const auto *e = lookup(kF64, L);

@@ -22,6 +22,7 @@
 // reduction of the angle and rounded once to the working precision; the
 // four-step and real-FFT twiddles come from two-level FP64 tables multiplied
 // in FP64 on the device, then rounded once.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -516,6 +517,11 @@ struct Launch {
 // waits for the plan's previous work; calls on one stream are ordered by the
 // stream itself.  (genFFT's plans are reentrant across threads for distinct
 // buffers, FFTLevel.h:76-92; this keeps the shared-workspace plans so.)
+// The event only orders device work (hipStreamWaitEvent) and is never waited
+// on by the host, so it is recorded without the system-scope fence: with the
+// fence, each record wrote back and invalidated the caches, and the next
+// kernel on the stream started ~5.6 us after the plan's last one ended (MI355X
+// kernel traces, every call of a plan with a workspace).
 struct StreamOrder {
     std::mutex mu;
     hipEvent_t done = nullptr;
@@ -529,7 +535,7 @@ struct StreamOrder {
     hipError_t before(hipStream_t s)
     {
         if (!done) {
-            hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+            hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming | hipEventDisableSystemFence);
             if (e != hipSuccess) return e;
         }
         if (used && last != s) return hipStreamWaitEvent(s, done, 0);
@@ -538,9 +544,25 @@ struct StreamOrder {
     // call with mu held, after the plan's launches on s
     hipError_t after(hipStream_t s)
     {
+        mark(s);
+        return hipEventRecord(done, s);
+    }
+    // instead of after(), when the plan's last launch on s carried `done` as
+    // its stop event (hipExtLaunchKernel): the event then completes with that
+    // kernel, and no marker packet sits between it and the stream's next
+    // kernel (one cost ~2 us per call on MI355X)
+    void mark(hipStream_t s)
+    {
         last = s;
         used = true;
-        return hipEventRecord(done, s);
+    }
+    // the event to attach to the last launch on s; null while s is being
+    // captured into a graph (the capture then records it as a node, after())
+    hipEvent_t stop_event(hipStream_t s) const
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+        return done;
     }
 };
 
@@ -1727,7 +1749,8 @@ int describe_launches(const std::vector<Launch> &launches, const std::string &he
 // input multiply transform b's values by scale[b] on their store.
 int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, const void *in, void *out,
                  void *scratch, int direction, hipStream_t s, hipEvent_t *events = nullptr,
-                 const float *scale = nullptr, int64_t scale_div = 1);
+                 const float *scale = nullptr, int64_t scale_div = 1, hipEvent_t stop = nullptr,
+                 bool *stopped = nullptr);
 
 // Whether this call's buffers overlap in a way the plan's launch order can
 // corrupt (see gfft_plan_s::overlap_hazard).
@@ -1764,16 +1787,18 @@ int run(gfft_plan_s *p, const void *in, void *out, int direction, hipStream_t s,
         HIP_TRY(hipMemcpyAsync(p->stage, in, p->in_bytes, hipMemcpyDeviceToDevice, s));
         in = p->stage;
     }
+    bool stopped = false;
     int st = run_launches(p->launches, p->prec, p->xcd_remap, in, out, p->scratch, direction, s, events, scale,
-                          p->scale_div);
+                          p->scale_div, p->order.stop_event(s), &stopped);
     if (st) return st;
-    HIP_TRY(p->order.after(s));
+    if (stopped) p->order.mark(s);
+    else HIP_TRY(p->order.after(s));
     return GFFT_OK;
 }
 
 int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, const void *in, void *out,
                  void *scratch, int direction, hipStream_t s, hipEvent_t *events, const float *scale,
-                 int64_t scale_div)
+                 int64_t scale_div, hipEvent_t stop, bool *stopped)
 {
     struct {
         int prec, xcd_remap;
@@ -1783,6 +1808,9 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
     int li = 0;
     for (const Launch &L0 : launches) {
         Launch l = L0;
+        // `stop` completes with the last launch when that is a tile kernel
+        // (*stopped says so); other kinds leave it to the caller
+        hipEvent_t tail = stop && &L0 == &launches.back() ? stop : nullptr;
         if (events) HIP_TRY(hipEventRecord(events[2 * li], s));
         char *ip = bufs[l.in_buf] + l.in_off;
         char *op = bufs[l.out_buf] + l.out_off;
@@ -1805,7 +1833,9 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
                 a.scale_off = l.in_off / (int64_t)csize_of(l.k->prec, l.k->store);
             }
             void *args[] = {&a};
-            HIP_TRY(hipLaunchKernel((const void *)l.k->fn, dim3((unsigned)l.grid), dim3(l.k->NT), args, 0, s));
+            HIP_TRY(hipExtLaunchKernel((const void *)l.k->fn, dim3((unsigned)l.grid), dim3(l.k->NT), args, 0, s,
+                                       nullptr, tail, 0));
+            if (tail && stopped) *stopped = true;
             break;
         }
         case LT_DUAL: {
@@ -1826,7 +1856,9 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
                               l.first_b, l.last_b);
             int split = (int)l.split;
             void *args[] = {&a, &b, &split};
-            HIP_TRY(hipLaunchKernel((const void *)l.dual->fn, dim3((unsigned)l.grid), dim3(l.dual->NT), args, 0, s));
+            HIP_TRY(hipExtLaunchKernel((const void *)l.dual->fn, dim3((unsigned)l.grid), dim3(l.dual->NT), args, 0,
+                                       s, nullptr, tail, 0));
+            if (tail && stopped) *stopped = true;
             break;
         }
         case LT_UNPACK:
@@ -3067,8 +3099,10 @@ int gfft_filter_backward(gfft_filter_plan plan, const void *x, const void *H, co
     HIP_TRY(plan->order.before(s));
     HIP_TRY(launch_filter(plan->prec, 1, a, s));
     a.y = gH;
-    HIP_TRY(launch_filter(plan->prec, 2, a, s));
-    HIP_TRY(plan->order.after(s));
+    const hipEvent_t stop = plan->order.stop_event(s);
+    HIP_TRY(launch_filter(plan->prec, 2, a, s, stop));
+    if (stop) plan->order.mark(s);
+    else HIP_TRY(plan->order.after(s));
     return GFFT_OK;
 }
 
