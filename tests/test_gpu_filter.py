@@ -182,6 +182,38 @@ def test_fused_matches_unfused(gpu, prec):
         assert O.rel_l2(a, r) <= 4 * tol
 
 
+def test_filter_backward_on_several_streams(gpu):
+    """The backward's per-row gH partials live in the plan, shared by every
+    call: calls on three streams, enqueued back to back with no host
+    synchronisation, each give the gradients of a call run alone."""
+    import genfft_amd as G
+    g = torch.Generator(device=gpu).manual_seed(5)
+    n, b = 16384, 256
+    xs = [torch.randn(b, n, device=gpu, generator=g) for _ in range(3)]
+    gys = [torch.randn(b, n, device=gpu, generator=g) for _ in range(3)]
+    H = (1 + 0.01 * torch.randn(n // 2 + 1, device=gpu, generator=g)).to(torch.complex64)
+
+    def grads(x, gy):
+        tx, tH = x.clone().requires_grad_(True), H.clone().requires_grad_(True)
+        G.spectral_filter(tx, tH).backward(gy)
+        return tx.grad, tH.grad
+
+    ref = []
+    for x, gy in zip(xs, gys):
+        ref.append(grads(x, gy))
+        torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(device=gpu) for _ in range(3)]
+    for rep in range(2):
+        got = []
+        for x, gy, s in zip(xs, gys, streams):
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                got.append(grads(x, gy))
+        torch.cuda.synchronize()
+        for (gx, gH), (rx, rH) in zip(got, ref):
+            assert torch.equal(gx, rx) and torch.equal(gH, rH), rep
+
+
 def test_filter_in_place_and_batch_shapes(gpu):
     """Leading axes are flattened into rows; a plan is reused across calls."""
     import genfft_amd as G
