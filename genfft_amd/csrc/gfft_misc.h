@@ -6,6 +6,16 @@
 
 namespace gfft {
 
+// A one-argument kernel launch whose completion also completes `stop` (when
+// given): the event needs no marker packet of its own after the kernel.
+template <class A>
+inline hipError_t launch_stop(void (*fn)(A), dim3 grid, dim3 block, hipStream_t s, A a, hipEvent_t stop)
+{
+    void *args[] = {&a};
+    return hipExtLaunchKernel((const void *)fn, grid, block, args, 0, s, nullptr, stop, 0);
+}
+
+
 // Two-level FP64 table of W_N^m = exp(-2*pi*i*m/N):
 //   W_N^m = hi[m >> lo_bits] * lo[m & ((1 << lo_bits) - 1)]
 struct TwTable {
@@ -103,11 +113,11 @@ struct BitrevArgs {
 };
 hipError_t launch_bitrev(int prec, const BitrevArgs &a, hipStream_t s);
 
-hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s);
+hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s, hipEvent_t stop = nullptr);
 hipError_t launch_pack(int prec, const UnpackArgs &a, hipStream_t s);
 hipError_t launch_separate(int prec, const SeparateArgs &a, hipStream_t s);
 hipError_t launch_interleave(int prec, const InterleaveArgs &a, hipStream_t s);
-hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s);
+hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s, hipEvent_t stop = nullptr);
 hipError_t launch_copy(int prec, const CopyArgs &a, hipStream_t s);
 hipError_t launch_repack(const RepackArgs &a, hipStream_t s);
 

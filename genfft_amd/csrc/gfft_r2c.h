@@ -73,7 +73,7 @@ struct R2cArgs {
 // tile-kernel function, only the shape (for the pass twiddle table).
 const KernelEntry *r2c_fused_entry(int prec);
 
-hipError_t launch_r2c_fused(int prec, const R2cArgs &a, hipStream_t s);  // one workgroup per tile
+hipError_t launch_r2c_fused(int prec, const R2cArgs &a, hipStream_t s, hipEvent_t stop = nullptr);  // one workgroup per tile
 
 // The inverse (c2r) mirror image: the pack Z_k = (F_k + conj F_{L-k})
 // + i W_n^-k (F_k - conj F_{L-k}) fused into the first four-step pass of the
@@ -112,6 +112,7 @@ enum RealRowsMode { RR_R2C = 0, RR_C2R = 1 };
 
 // Shape for n/2 = L (the row-tile shape of that length; nullptr: none).
 const KernelEntry *real_rows_entry(int prec, int L);
-hipError_t launch_real_rows(int prec, int L, int mode, const RealRowsArgs &a, int64_t grid, hipStream_t s);
+hipError_t launch_real_rows(int prec, int L, int mode, const RealRowsArgs &a, int64_t grid, hipStream_t s,
+                            hipEvent_t stop = nullptr);
 
 }  // namespace gfft

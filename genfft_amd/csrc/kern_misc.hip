@@ -366,12 +366,10 @@ hipError_t launch_bitrev(int prec, const BitrevArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s)
+hipError_t launch_unpack(int prec, const UnpackArgs &a, hipStream_t s, hipEvent_t stop)
 {
     const unsigned g = grid_for(((a.n >> 2) + 1) * a.batch);
-    if (prec) hipLaunchKernelGGL(unpack_kernel<double>, dim3(g), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(unpack_kernel<float>, dim3(g), dim3(256), 0, s, a);
-    return hipGetLastError();
+    return launch_stop(prec ? &unpack_kernel<double> : &unpack_kernel<float>, dim3(g), dim3(256), s, a, stop);
 }
 
 hipError_t launch_pack(int prec, const UnpackArgs &a, hipStream_t s)
@@ -398,12 +396,10 @@ hipError_t launch_interleave(int prec, const InterleaveArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s)
+hipError_t launch_hfill(int prec, const HFillArgs &a, hipStream_t s, hipEvent_t stop)
 {
     const unsigned g = grid_for((a.M - a.vcols) * a.N * a.batch);
-    if (prec) hipLaunchKernelGGL(hfill_kernel<double>, dim3(g), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(hfill_kernel<float>, dim3(g), dim3(256), 0, s, a);
-    return hipGetLastError();
+    return launch_stop(prec ? &hfill_kernel<double> : &hfill_kernel<float>, dim3(g), dim3(256), s, a, stop);
 }
 
 hipError_t launch_repack(const RepackArgs &a, hipStream_t s)

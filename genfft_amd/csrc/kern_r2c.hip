@@ -161,10 +161,9 @@ struct Fused<double> {
 };
 
 template <typename T>
-hipError_t launch_t(const R2cArgs &a, hipStream_t s)
+hipError_t launch_t(const R2cArgs &a, hipStream_t s, hipEvent_t stop)
 {
-    hipLaunchKernelGGL(Fused<T>::kernel, dim3((unsigned)a.n_tiles), dim3(Fused<T>::entry.NT), 0, s, a);
-    return hipGetLastError();
+    return launch_stop(Fused<T>::kernel, dim3((unsigned)a.n_tiles), dim3(Fused<T>::entry.NT), s, a, stop);
 }
 
 // ---------------------------------------------------------------------------
@@ -325,9 +324,9 @@ const KernelEntry *r2c_fused_entry(int prec)
     return prec ? &Fused<double>::entry : &Fused<float>::entry;
 }
 
-hipError_t launch_r2c_fused(int prec, const R2cArgs &a, hipStream_t s)
+hipError_t launch_r2c_fused(int prec, const R2cArgs &a, hipStream_t s, hipEvent_t stop)
 {
-    return prec ? launch_t<double>(a, s) : launch_t<float>(a, s);
+    return prec ? launch_t<double>(a, s, stop) : launch_t<float>(a, s, stop);
 }
 
 const KernelEntry *c2r_fused_entry(int prec, int n1)

@@ -153,7 +153,8 @@ const KernelEntry *real_rows_entry(int prec, int L)
     return e ? &e->entry : nullptr;
 }
 
-hipError_t launch_real_rows(int prec, int L, int mode, const RealRowsArgs &a, int64_t grid, hipStream_t s)
+hipError_t launch_real_rows(int prec, int L, int mode, const RealRowsArgs &a, int64_t grid, hipStream_t s,
+                            hipEvent_t stop)
 {
     void (*fn)(RealRowsArgs) = nullptr;
     int nt = 0;
@@ -163,8 +164,7 @@ hipError_t launch_real_rows(int prec, int L, int mode, const RealRowsArgs &a, in
         if (const auto *e = rows_find(kRowsF32, L)) fn = e->fn[mode], nt = e->entry.NT;
     }
     if (!fn) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(nt), 0, s, a);
-    return hipGetLastError();
+    return launch_stop(fn, dim3((unsigned)grid), dim3(nt), s, a, stop);
 }
 
 }  // namespace gfft

@@ -1808,8 +1808,9 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
     int li = 0;
     for (const Launch &L0 : launches) {
         Launch l = L0;
-        // `stop` completes with the last launch when that is a tile kernel
-        // (*stopped says so); other kinds leave it to the caller
+        // `stop` completes with the last launch when that is a tile, real-row,
+        // fused r2c, unpack or Hermitian-fill kernel (*stopped says so); other
+        // kinds leave it to the caller
         hipEvent_t tail = stop && &L0 == &launches.back() ? stop : nullptr;
         if (events) HIP_TRY(hipEventRecord(events[2 * li], s));
         char *ip = bufs[l.in_buf] + l.in_off;
@@ -1866,13 +1867,15 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             UnpackArgs a = l.ua;
             a.in = ip;
             a.out = op;
-            HIP_TRY(l.type == LT_UNPACK ? launch_unpack(p->prec, a, s) : launch_pack(p->prec, a, s));
+            HIP_TRY(l.type == LT_UNPACK ? launch_unpack(p->prec, a, s, tail) : launch_pack(p->prec, a, s));
+            if (tail && stopped && l.type == LT_UNPACK) *stopped = true;
             break;
         }
         case LT_HFILL: {
             HFillArgs a = l.ha;
             a.out = op;
-            HIP_TRY(launch_hfill(p->prec, a, s));
+            HIP_TRY(launch_hfill(p->prec, a, s, tail));
+            if (tail && stopped) *stopped = true;
             break;
         }
         case LT_RROWS: {
@@ -1882,7 +1885,8 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             a.t.vec_in = l.vin_ok && (((uintptr_t)ip & 15) == 0);
             a.t.vec_out = l.vout_ok && (((uintptr_t)op & 15) == 0);
             a.t.xcd_remap = p->xcd_remap;
-            HIP_TRY(launch_real_rows(p->prec, l.k->L, l.rr_mode, a, l.grid, s));
+            HIP_TRY(launch_real_rows(p->prec, l.k->L, l.rr_mode, a, l.grid, s, tail));
+            if (tail && stopped) *stopped = true;
             break;
         }
         case LT_C2R: {
@@ -1899,7 +1903,8 @@ int run_launches(const std::vector<Launch> &launches, int prec, int xcd_remap, c
             a.out = op;
             a.vec_in = l.vin_ok && (((uintptr_t)ip & 15) == 0);
             a.xcd_remap = p->xcd_remap;
-            HIP_TRY(launch_r2c_fused(p->prec, a, s));
+            HIP_TRY(launch_r2c_fused(p->prec, a, s, tail));
+            if (tail && stopped) *stopped = true;
             break;
         }
         case LT_COPY: {
